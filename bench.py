@@ -1,0 +1,199 @@
+"""Benchmark of the tcuscan hot path on one MI355X.
+
+Default workload: the large 1-D scan, N = 2^28 fp16 values uniform in
+[-1, 1] (fixed seed) -> float32 inclusive prefix sums through
+tcuscan.run_scan_multi_core(x, S=128).  A step is one call over the whole
+input with the input already resident in HBM.  The metric is GB/s of
+algorithmic traffic, N * (2 + 4) bytes per step (the reference's formula).
+
+Printed: one JSON line with the driver's fields plus
+  roofline      the scan kernel's average duration from HIP events on the
+                stream it runs on, as achieved GB/s against 8 TB/s;
+  cpu_baseline  the oracle's C port of the reference's run_scan_cpu
+                (three-phase OpenMP blocked scan), timed on this host over
+                whole copies of the same workload;
+  extras        the other large-scan workloads (int8 -> int32, bf16 -> fp32)
+                measured the same way.
+`--gpus N` with N > 1 exits: this build runs on one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(REPO, "pytorch-tcuscan_amd")
+sys.path.insert(0, PKG)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E peak
+
+
+def _timed_steps(fn, steps: int, warmup: int):
+    """Runs fn warmup+steps times; returns (wall seconds for the timed steps,
+    per-step event milliseconds)."""
+    import torch
+
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    t0 = time.perf_counter()
+    for i in range(steps):
+        starts[i].record()
+        fn()
+        ends[i].record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return wall, [s.elapsed_time(e) for s, e in zip(starts, ends)]
+
+
+def _scan_workload(dtype_name: str, n: int, S: int, steps: int, warmup: int):
+    import torch
+
+    import tcuscan
+
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device="cpu").manual_seed(0)
+    if dtype_name == "int8":
+        x = torch.randint(-4, 5, (n,), generator=g, dtype=torch.int8).to(dev)
+        in_b, out_b = 1, 4
+    else:
+        dt = torch.float16 if dtype_name == "fp16" else torch.bfloat16
+        x = (torch.rand(n, generator=g) * 2 - 1).to(dt).to(dev)
+        in_b, out_b = 2, 4
+    torch.cuda.synchronize()
+    wall, ev_ms = _timed_steps(lambda: tcuscan.run_scan_multi_core(x, S), steps, warmup)
+    ms = sorted(ev_ms)
+    avg_ms = sum(ev_ms) / len(ev_ms)
+    bytes_per_step = n * (in_b + out_b)
+    del x
+    torch.cuda.empty_cache()
+    return {
+        "bytes_per_step": bytes_per_step,
+        "wall_s": wall,
+        "event_avg_ms": avg_ms,
+        "event_p50_ms": ms[len(ms) // 2],
+        "event_p99_ms": ms[min(len(ms) - 1, int(0.99 * len(ms)))],
+        "gbs_wall": bytes_per_step * steps / wall / 1e9,
+        "gbs_kernel": bytes_per_step / (avg_ms * 1e-3) / 1e9,
+    }
+
+
+def _cpu_baseline(n: int, budget_s: float = 10.0):
+    """Times the oracle's C port of run_scan_cpu on the full fp16 workload."""
+    import numpy as np
+
+    lib_path = os.path.join(REPO, "oracle", "libscan_cpu_port.so")
+    if not os.path.exists(lib_path):
+        import subprocess
+        subprocess.run(["make", "-s", "-C", os.path.join(REPO, "oracle")], check=True)
+    lib = ctypes.CDLL(lib_path)
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
+    threads = max(1, min(threads, os.cpu_count() or threads))
+    x = ((np.random.default_rng(0).random(n, dtype=np.float32) * 2) - 1).astype(np.float16)
+    y = np.empty(n, np.float32)
+    args = (x.ctypes.data_as(ctypes.c_void_p), y.ctypes.data_as(ctypes.c_void_p), ctypes.c_int64(n), threads)
+    lib.port_scan_cpu_f16(*args)  # warm (page-in)
+    reps, t0 = 0, time.perf_counter()
+    while True:
+        lib.port_scan_cpu_f16(*args)
+        reps += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s or reps >= 200:
+            break
+    per = el / reps
+    return {
+        "value": round(n * 6 / per / 1e9, 3),
+        "unit": "GB/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{reps} full runs of the N=2^{n.bit_length() - 1} fp16->fp32 scan "
+                  f"(run_scan_cpu semantics, OpenMP three-phase blocked scan), {per * 1e3:.1f} ms each",
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--log2n", type=int, default=28)
+    ap.add_argument("--S", type=int, default=128)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-extras", action="store_true", help="skip int8/bf16 extras")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported", file=sys.stderr)
+        return 2
+    if int(os.environ.get("RANK", "0")) != 0:
+        return 0
+
+    n = 1 << a.log2n
+    main_res = _scan_workload("fp16", n, a.S, a.steps, a.warmup)
+    extras = {}
+    if not a.no_extras:
+        for name in ("int8", "bf16"):
+            r = _scan_workload(name, n, a.S, max(10, a.steps // 2), max(3, a.warmup // 2))
+            extras[f"scan_{name}_2p{a.log2n}"] = {
+                "value": round(r["gbs_wall"], 2), "unit": "GB/s",
+                "kernel_gbs": round(r["gbs_kernel"], 2),
+                "frac_of_peak": round(r["gbs_kernel"] / HBM_PEAK_GBS, 4),
+                "ms_per_step": round(r["wall_s"] * 1e3 / max(10, a.steps // 2), 4),
+            }
+    cpu = None if a.no_cpu else _cpu_baseline(n)
+
+    ms_per_step = main_res["wall_s"] * 1e3 / a.steps
+    out = {
+        "metric": "scan GB/s & %HBM peak (N=2^28 fp16/int8); SpMV GFLOP/s; p50/p99 mixed-stream",
+        "value": round(main_res["gbs_wall"], 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp16->f32 (f64 accumulate)",
+        "data": "synthetic: U[-1,1] rounded to fp16, torch.Generator seed 0",
+        "config": {"workload": f"large 1-D inclusive scan, N=2^{a.log2n} fp16 -> fp32, S={a.S}",
+                   "op": "tcuscan.run_scan_multi_core", "N": n, "S": a.S,
+                   "bytes_per_step": main_res["bytes_per_step"]},
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(main_res["gbs_kernel"], 2),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(main_res["gbs_kernel"] / HBM_PEAK_GBS, 4),
+            "traffic": _pmc_traffic(),
+            "kernel_avg_ms": round(main_res["event_avg_ms"], 4),
+            "kernel_p50_ms": round(main_res["event_p50_ms"], 4),
+            "kernel_p99_ms": round(main_res["event_p99_ms"], 4),
+        },
+        "cpu_baseline": cpu,
+        "extras": extras,
+    }
+    print(json.dumps(out))
+    return 0
+
+
+def _pmc_traffic():
+    """HBM bytes per launch of the scan kernel from the committed rocprofv3
+    --pmc summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel
+    calibration factor recorded beside them), or None when absent."""
+    path = os.path.join(REPO, "profiles", "scan_f16_2p28_pmc.json")
+    try:
+        with open(path) as f:
+            d = json.load(f)
+        return d.get("traffic_bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,80 @@
+"""CPU tests of the C ABI: the library loads, exports every declared entry
+point, and rejects invalid arguments before anything reaches the GPU."""
+import ctypes
+import os
+import re
+
+import pytest
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(REPO, "include", "tcuscan.h")
+LIB = os.path.join(REPO, "pytorch-tcuscan_amd", "libtcuscan_hip.so")
+
+TCS_OK, TCS_ERR_INVALID_ARG, TCS_ERR_UNSUPPORTED_DTYPE, TCS_ERR_WORKSPACE = 0, 1, 2, 3
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(tcs_[a-z0-9_]+)\s*\(", text)))
+
+
+@pytest.fixture(scope="module")
+def lib():
+    assert os.path.exists(LIB), "libtcuscan_hip.so is not built"
+    return ctypes.CDLL(LIB)
+
+
+def test_exports_every_declared_symbol(lib):
+    names = declared_functions()
+    assert len(names) >= 5
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, f"declared but not exported: {missing}"
+
+
+def test_status_strings_and_version(lib):
+    lib.tcs_status_string.restype = ctypes.c_char_p
+    assert lib.tcs_status_string(0) == b"ok"
+    assert b"dtype" in lib.tcs_status_string(2)
+    assert lib.tcs_abi_version() >= 100
+
+
+def _scan(lib, x, dtype, y, n, S, ws, ws_bytes):
+    f = lib.tcs_scan
+    f.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int,
+                  ctypes.c_int, ctypes.c_double, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]
+    return f(x, dtype, y, n, S, 0, 0.0, ws, ws_bytes, None)
+
+
+def test_scan_argument_validation(lib):
+    lib.tcs_scan_workspace_bytes.restype = ctypes.c_size_t
+    lib.tcs_scan_workspace_bytes.argtypes = [ctypes.c_int64, ctypes.c_int, ctypes.c_int]
+    need = lib.tcs_scan_workspace_bytes(1 << 20, 128, 0)
+    assert need >= 256 + 16 * ((1 << 20) // 16384)
+    fake = 0x100000  # aligned, never dereferenced: every call below fails validation
+    assert _scan(lib, fake, 0, fake, 0, 128, None, 0) == TCS_OK  # empty input: no launch
+    assert _scan(lib, fake, 0, fake, -1, 128, fake, need) == TCS_ERR_INVALID_ARG
+    assert _scan(lib, fake, 0, fake, 1 << 20, 0, fake, need) == TCS_ERR_INVALID_ARG
+    assert _scan(lib, None, 0, fake, 1 << 20, 128, fake, need) == TCS_ERR_INVALID_ARG
+    assert _scan(lib, fake + 2, 0, fake, 1 << 20, 128, fake, need) == TCS_ERR_INVALID_ARG
+    assert _scan(lib, fake, 0, fake, 1 << 20, 128, fake, need - 16) == TCS_ERR_WORKSPACE
+    assert _scan(lib, fake, 0, fake, 1 << 20, 128, None, need) == TCS_ERR_WORKSPACE
+    assert _scan(lib, fake, 99, fake, 1 << 20, 128, fake, need) == TCS_ERR_UNSUPPORTED_DTYPE
+    assert _scan(lib, fake, 7, fake, 1 << 20, 128, fake, need) == TCS_ERR_UNSUPPORTED_DTYPE
+
+
+def test_scan_rows_argument_validation(lib):
+    f = lib.tcs_scan_rows
+    f.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64,
+                  ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]
+    fake = 0x100000
+    assert f(fake, 0, fake, 1000, 0, 64, 0, fake, 1 << 20, None) == TCS_ERR_INVALID_ARG
+    assert f(fake, 42, fake, 1000, 10, 64, 0, fake, 1 << 20, None) == TCS_ERR_UNSUPPORTED_DTYPE
+
+
+def test_extension_rejects_cpu_tensors():
+    import torch
+    import tcuscan_ops
+
+    with pytest.raises(RuntimeError, match="GPU tensor"):
+        tcuscan_ops.run_scan_multi_core(torch.ones(4, dtype=torch.float16), 64)
