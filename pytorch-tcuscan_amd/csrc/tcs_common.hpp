@@ -18,48 +18,69 @@ typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 // fp16/bf16/fp32 accumulate in double and emit float; int8/int32 accumulate
 // and emit int32 (two's-complement wrap, as torch.cumsum(dtype=int32)).
 // ---------------------------------------------------------------------------
+// G is the group width: contiguous elements one lane loads per group
+// (16 bytes of fp16/bf16/fp32 input, 8 bytes of int8).
 struct CodecF16 {
+  static constexpr int G = 8;
   using In = uint16_t; using Acc = double; using Out = float;
   __device__ static Acc conv(In v) { return to_acc_f(v, F16{}); }
   __device__ static Out out(Acc a) { return (float)a; }
 };
 struct CodecBF16 {
+  static constexpr int G = 8;
   using In = uint16_t; using Acc = double; using Out = float;
   __device__ static Acc conv(In v) { return to_acc_f(v, BF16{}); }
   __device__ static Out out(Acc a) { return (float)a; }
 };
 struct CodecF32 {
+  static constexpr int G = 4;
   using In = float; using Acc = double; using Out = float;
   __device__ static Acc conv(In v) { return (double)v; }
   __device__ static Out out(Acc a) { return (float)a; }
 };
 struct CodecI8 {
+  static constexpr int G = 8;
   using In = int8_t; using Acc = int; using Out = int;
   __device__ static Acc conv(In v) { return (int)v; }
   __device__ static Out out(Acc a) { return a; }
 };
 struct CodecI32 {
+  static constexpr int G = 4;
   using In = int; using Acc = int; using Out = int;
   __device__ static Acc conv(In v) { return v; }
   __device__ static Out out(Acc a) { return a; }
 };
 
 // ---------------------------------------------------------------------------
-// Per-thread contiguous loads/stores of ITEMS elements.  Full tiles use
-// 16-byte vector accesses; the ragged last tile falls back to guarded scalar
-// accesses.  Pointers handed to the kernels are 16-byte aligned (checked on
-// the host).
+// Per-lane loads/stores of ITEMS contiguous elements.  Full groups use 16-,
+// 8- or 4-byte vector accesses; a ragged group falls back to guarded scalar
+// accesses.  The kernels stripe these groups across the 64 lanes so that one
+// wave instruction covers one contiguous span (a 16-byte group per lane ->
+// 1 KiB per instruction).  Plain (temporal) stores are used: partial-line
+// nontemporal stores measured 2.5x slower on gfx950 for 32-byte lane groups.
+// Pointers handed to the kernels are 16-byte aligned (checked on the host).
 // ---------------------------------------------------------------------------
-template <class T, int ITEMS>
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+
+// FULL = true asserts that the whole group is in range (the caller checked
+// it once per tile), so only the vector path is emitted: mixing the guarded
+// scalar path into the same code serialises the loads behind waitcnts.
+template <class T, int ITEMS, bool FULL = false>
 __device__ __forceinline__ void load_items(const T* __restrict__ p, int64_t base, int64_t n, T (&r)[ITEMS]) {
   constexpr int kBytes = ITEMS * (int)sizeof(T);
-  if (base + ITEMS <= n) {
+  if (FULL || base + ITEMS <= n) {
     if constexpr (kBytes % 16 == 0) {
       const u32x4* vp = reinterpret_cast<const u32x4*>(p + base);
       u32x4 tmp[kBytes / 16];
 #pragma unroll
-      for (int k = 0; k < kBytes / 16; ++k) tmp[k] = __builtin_nontemporal_load(vp + k);
+      for (int k = 0; k < kBytes / 16; ++k) tmp[k] = vp[k];
       __builtin_memcpy(r, tmp, kBytes);
+    } else if constexpr (kBytes == 8) {
+      const u32x2 v = *reinterpret_cast<const u32x2*>(p + base);
+      __builtin_memcpy(r, &v, 8);
+    } else if constexpr (kBytes == 4) {
+      const uint32_t v = *reinterpret_cast<const uint32_t*>(p + base);
+      __builtin_memcpy(r, &v, 4);
     } else {
 #pragma unroll
       for (int i = 0; i < ITEMS; ++i) r[i] = p[base + i];
@@ -70,16 +91,20 @@ __device__ __forceinline__ void load_items(const T* __restrict__ p, int64_t base
   }
 }
 
-template <class T, int ITEMS>
+template <class T, int ITEMS, bool FULL = false>
 __device__ __forceinline__ void store_items(T* __restrict__ p, int64_t base, int64_t n, const T (&r)[ITEMS]) {
   constexpr int kBytes = ITEMS * (int)sizeof(T);
-  if (base + ITEMS <= n) {
+  if (FULL || base + ITEMS <= n) {
     if constexpr (kBytes % 16 == 0) {
       u32x4 tmp[kBytes / 16];
       __builtin_memcpy(tmp, r, kBytes);
       u32x4* vp = reinterpret_cast<u32x4*>(p + base);
 #pragma unroll
-      for (int k = 0; k < kBytes / 16; ++k) __builtin_nontemporal_store(tmp[k], vp + k);
+      for (int k = 0; k < kBytes / 16; ++k) vp[k] = tmp[k];
+    } else if constexpr (kBytes == 8) {
+      u32x2 v;
+      __builtin_memcpy(&v, r, 8);
+      *reinterpret_cast<u32x2*>(p + base) = v;
     } else {
 #pragma unroll
       for (int i = 0; i < ITEMS; ++i) p[base + i] = r[i];
@@ -89,6 +114,18 @@ __device__ __forceinline__ void store_items(T* __restrict__ p, int64_t base, int
     for (int i = 0; i < ITEMS; ++i)
       if (base + i < n) p[base + i] = r[i];
   }
+}
+
+// Marks the raw words of `r` as rewritten so the compiler recomputes values
+// derived from them instead of keeping those (wider) values live in VGPRs.
+template <class T, int ITEMS>
+__device__ __forceinline__ void reg_barrier(T (&r)[ITEMS]) {
+  constexpr int kWords = (ITEMS * (int)sizeof(T) + 3) / 4;
+  uint32_t w[kWords] = {};
+  __builtin_memcpy(w, r, ITEMS * sizeof(T));
+#pragma unroll
+  for (int k = 0; k < kWords; ++k) asm volatile("" : "+v"(w[k]));
+  __builtin_memcpy(r, w, ITEMS * sizeof(T));
 }
 
 // ---------------------------------------------------------------------------
@@ -109,20 +146,27 @@ __device__ __forceinline__ uint64_t* ws_desc(void* ws) {
 }
 
 // Tile length (elements) for a look-back kernel: tiles_per_wg matrix tiles
-// of S*S elements, clamped to the instantiated range [1024, 16384] and
-// rounded down to a power of two.  tiles_per_wg <= 0 selects the default.
+// of S*S elements, clamped to the instantiated range [1024, 65536] and
+// rounded down to a power of two.  tiles_per_wg <= 0 selects the default
+// tile of 32768 elements: large enough that the per-tile ticket atomic
+// (about 12 ns, serialised on one address) stays off the critical path.
+constexpr int64_t kDefaultTile = 32768;
 inline int64_t tile_elems_for(int S, int tiles_per_wg) {
   int64_t s2 = (int64_t)S * S;
-  if (tiles_per_wg <= 0) tiles_per_wg = s2 >= 16384 ? 1 : (int)((16384 + s2 - 1) / s2);
+  if (tiles_per_wg <= 0) tiles_per_wg = (int)((kDefaultTile + s2 - 1) / s2);
   int64_t t = s2 * tiles_per_wg;
   if (t < 1024) t = 1024;
-  if (t > 16384) t = 16384;
+  if (t > 65536) t = 65536;
   int64_t p = 1024;
   while (p * 2 <= t) p *= 2;
   return p;
 }
 
 inline bool aligned16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
+
+// Launch tag for the next look-back launch on workspace `ws` (one counter
+// per workspace address, kept on the host; see tag_of_epoch).
+uint32_t next_launch_tag(const void* ws);
 
 #define TCS_CHECK_LAUNCH()                                  \
   do {                                                      \
