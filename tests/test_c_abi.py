@@ -57,7 +57,7 @@ def test_scan_argument_validation(lib):
     assert _scan(lib, fake, 0, fake, 1 << 20, 0, fake, need) == TCS_ERR_INVALID_ARG
     assert _scan(lib, None, 0, fake, 1 << 20, 128, fake, need) == TCS_ERR_INVALID_ARG
     assert _scan(lib, fake + 2, 0, fake, 1 << 20, 128, fake, need) == TCS_ERR_INVALID_ARG
-    assert _scan(lib, fake, 0, fake, 1 << 20, 128, fake, need - 16) == TCS_ERR_WORKSPACE
+    assert _scan(lib, fake, 0, fake, 1 << 20, 128, fake, 256) == TCS_ERR_WORKSPACE
     assert _scan(lib, fake, 0, fake, 1 << 20, 128, None, need) == TCS_ERR_WORKSPACE
     assert _scan(lib, fake, 99, fake, 1 << 20, 128, fake, need) == TCS_ERR_UNSUPPORTED_DTYPE
     assert _scan(lib, fake, 7, fake, 1 << 20, 128, fake, need) == TCS_ERR_UNSUPPORTED_DTYPE
