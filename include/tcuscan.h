@@ -101,6 +101,118 @@ int tcs_seg_scan_flags(const void* x, int x_dtype, const int8_t* flags, void* y,
                        int S, int tiles_per_wg, void* workspace, size_t workspace_bytes,
                        void* stream);
 
+
+/* ------------------------------------------------------------------------
+ * Reduction and down-sweep building blocks
+ *
+ * These keep the reference's block geometry.  reduce_tiles / complete_rows
+ * split x into blocks [b*BL, min((b+1)*BL, n)) with BL = T * ceil(ceil(n/T)
+ * / num_blocks) for a tile of T elements (scalar::GetWorkDistribution,
+ * src/kernels/tcuscan_utils.h:1389-1408).  Every workspace's first 256 bytes
+ * are reserved (zero between calls); *_workspace_bytes include them.
+ * --------------------------------------------------------------------- */
+
+size_t tcs_reduce_tiles_workspace_bytes(int64_t n, int64_t tile_len, int64_t num_blocks);
+
+/* out[b] = sum of block b (T = tile_len).  x: F16, BF16, F32 -> float32;
+ * I8, I32 -> int32 (exact).  Deterministic (fixed-order partials).
+ * Replaces: reduce_tiles_{fp16,int8} (src/reduce_tiles.cpp:48,70) behind
+ * tcuscan::run_reduce_tiles (src/torch/torch_reduce.h:42). */
+int tcs_reduce_tiles(const void* x, int x_dtype, void* out, int64_t n, int64_t tile_len,
+                     int64_t num_blocks, void* workspace, size_t workspace_bytes, void* stream);
+
+size_t tcs_complete_blocks_workspace_bytes(int64_t num_blocks);
+
+/* y[i] = x[i] + sum(sums[:b]) with b = i / (n / num_blocks); n must be a
+ * multiple of num_blocks.  dtype: F32 or I32 for x, sums and y.
+ * Replaces: complete_blocks_{fp32,int32} (src/complete_blocks.cpp:56,75)
+ * behind run_complete_blocks (src/torch/torch_reduce.h:197). */
+int tcs_complete_blocks(const void* x, int dtype, const void* sums, int64_t num_blocks, void* y,
+                        int64_t n, int64_t tile_len, void* workspace, size_t workspace_bytes,
+                        void* stream);
+
+size_t tcs_complete_rows_workspace_bytes(int64_t n, int64_t num_sums, int64_t tile_width,
+                                         int64_t tile_height);
+
+/* Down-sweep of the multi-core scan: blocks as above with T = tile_width *
+ * min(tile_height, ceil(8192 / tile_width)); inside block b, rows of
+ * tile_width elements get a carry starting at sum(sums[:b]), and each row's
+ * last output becomes the next row's carry.  dtype: F32 or I32.
+ * Replaces: complete_rows_{fp32,int32} (src/complete_rows.cpp:53,72) behind
+ * run_complete_rows (src/torch/torch_reduce.h:143). */
+int tcs_complete_rows(const void* x, int dtype, const void* sums, int64_t num_sums, void* y,
+                      int64_t n, int64_t tile_width, int64_t tile_height, void* workspace,
+                      size_t workspace_bytes, void* stream);
+
+/* ------------------------------------------------------------------------
+ * Segmented sums and SpMV
+ * --------------------------------------------------------------------- */
+
+size_t tcs_seg_sum_workspace_bytes(int64_t nnz);
+
+/* y[r] = sum(x[end(r-1) : end(r)]) for r < num_segments, with end(-1) = 0,
+ * end(r) = ends[r] for r < num_ends and nnz otherwise (num_ends <=
+ * num_segments).  For a full CSR indptr pass ends = indptr + 1 and
+ * num_ends = num_segments; for the start-only convention of
+ * run_seg_sum_single_cube pass ends = starts + 1 and num_ends =
+ * num_segments - 1.  Empty segments give 0.  ends dtype: I32, U32, I64
+ * (non-decreasing).  x: F16, BF16, F32 -> float32 (double accumulation);
+ * I8 -> int32 (exact).  Rows spanning tiles are completed by look-back, so
+ * the result does not depend on how rows are distributed.
+ * Replaces: seg_sum_single_core_{fp16,int8}, seg_sum_multi_core_{fp16,int8},
+ * seg_sum_single_cube_fp16, seg_sum_multi_cube_fp16
+ * (src/seg_sum_single_core.cpp:30,60, src/seg_sum_multi_core.cpp:118,150,
+ * src/seg_sum_single_cube.cpp:79, src/seg_sum_multi_cube.cpp:115) behind
+ * run_seg_sum_single_core / _multi_core / _single_cube / _multi_cube
+ * (src/torch/torch_seg_ops.h:211,333,278,443). */
+int tcs_seg_sum_indptr(const void* x, int x_dtype, const void* ends, int ends_dtype,
+                       int64_t num_segments, int64_t num_ends, void* y, int64_t nnz,
+                       void* workspace, size_t workspace_bytes, void* stream);
+
+/* CSR SpMV y = A x: y[r] = sum over row r of vals[p] * x[cols[p]], rows
+ * delimited as in tcs_seg_sum_indptr with ends = indptr + 1 (num_rows
+ * entries).  vals/x: F16 or F32 (same dtype); products are formed exactly
+ * and summed in double, y is float32.  Columns outside [0, x_len)
+ * contribute 0.  Uses tcs_seg_sum_workspace_bytes(nnz).
+ * Replaces: spmv_v2_{fp16,fp32} (src/spmv_v2.cpp:126,162) behind
+ * run_spmv_v2 (src/torch/torch_spmv.h:112). */
+int tcs_spmv_csr(const void* vals, int vals_dtype, const void* indptr_ends, int ends_dtype,
+                 const int32_t* cols, const void* x, int64_t x_len, float* y, int64_t num_rows,
+                 int64_t nnz, void* workspace, size_t workspace_bytes, void* stream);
+
+/* z[i] = vals[i] * x[cols[i]] rounded to the input dtype (F16, F32: exact
+ * product rounded to nearest even; I16: two's-complement wrap).  Columns
+ * outside [0, x_len) give 0.
+ * Replaces: csr_gather_{fp16,int16,fp32} (src/csr_gather.cpp:5,16,27)
+ * behind run_csr_gather (src/torch/torch_gather.h:92). */
+int tcs_csr_gather(const void* vals, int dtype, const int32_t* cols, const void* x, int64_t x_len,
+                   void* z, int64_t nnz, void* stream);
+
+/* ------------------------------------------------------------------------
+ * Compaction
+ * --------------------------------------------------------------------- */
+
+size_t tcs_compress_workspace_bytes(int64_t n);
+
+/* *count_out (device int64) = number of i with mask[i] == 1.
+ * Replaces the sizing step of run_compress / run_compress_ind /
+ * _no_arange (run_reduce_tiles + .item(), src/torch/torch_compress.h:62-64). */
+int tcs_count_mask(const int8_t* mask, int64_t n, int64_t* count_out, void* workspace,
+                   size_t workspace_bytes, void* stream);
+
+/* Stable compaction of the elements with mask[i] == 1 into out_x (element
+ * size x_elem_bytes: 1, 2, 4 or 8; bits are copied).  idx_mode 0: values
+ * only; 1: also out_idx[k] = idx_in[i] (idx_elem_bytes 4 or 8); 2: also
+ * out_idx[k] = i (int32); 3: also out_idx[k] = i + 1 (int32).  At most
+ * out_capacity elements are written.  n < 2^31.
+ * Replaces: compress_{fp16,fp32}, compress_with_sums_*, compress_ind_*,
+ * compress_ind_no_arange_* (src/compress.cpp:26-242) behind run_compress,
+ * run_compress_pos, run_compress_ind, run_compress_ind_no_arange
+ * (src/torch/torch_compress.h:41,109,172,249). */
+int tcs_compress(const void* x, int x_elem_bytes, const void* idx_in, int idx_mode, int idx_elem_bytes,
+                 const int8_t* mask, void* out_x, void* out_idx, int64_t n, int64_t out_capacity,
+                 void* workspace, size_t workspace_bytes, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,339 @@
+// Stream compaction and CSR gather on gfx950.
+//
+// Compaction keeps x[i] where mask[i] == 1 exactly (the reference's
+// CompareScalar EQ 1, src/kernels/kernel_compress.h:167-168), in order.
+// One pass: each tile counts its kept elements (lane-serial, then DPP wave
+// scans and a block scan), the tile's output offset arrives through the
+// decoupled look-back on counts, and every lane writes its kept elements at
+// consecutive positions.  Optionally the kept positions (int32) or the
+// matching entries of an index array are written beside the values
+// (run_compress_ind / _no_arange).  A separate count pass sizes the outputs
+// when the caller does not know the count (the reference syncs on a reduce
+// for the same purpose, src/torch/torch_compress.h:62-67).
+//
+// CSR gather: z[i] = vals[i] * x[cols[i]] rounded once to the input dtype
+// (fp16/fp32: the exact product rounded to nearest even, as torch does;
+// int16: two's-complement wrap), following
+// src/kernels/kernel_csr_gather.h:122-137.  Columns outside [0, x_len)
+// yield 0 rather than an out-of-bounds read.
+#include "tcs_common.hpp"
+
+namespace tcs {
+
+constexpr int kCmpNW = 8, kCmpJ = 2, kCmpG = 8;
+constexpr int64_t kCmpTile = (int64_t)kCmpNW * kCmpJ * kWave * kCmpG;  // 8192
+
+template <int BYTES> struct Bits;
+template <> struct Bits<1> { using T = uint8_t; };
+template <> struct Bits<2> { using T = uint16_t; };
+template <> struct Bits<4> { using T = uint32_t; };
+template <> struct Bits<8> { using T = uint64_t; };
+
+// IDX: 0 = values only, 1 = gather idx_in, 2 = positions (int32),
+// 3 = positions + 1 (int32; the row ends of flag-delimited segments).
+template <int XB, int IB, int IDX>
+__global__ __launch_bounds__(kCmpNW * kWave) void compress_kernel(
+    const typename Bits<XB>::T* __restrict__ x, const typename Bits<IB>::T* __restrict__ idx_in,
+    const int8_t* __restrict__ mask, typename Bits<XB>::T* __restrict__ out_x,
+    typename Bits<IB>::T* __restrict__ out_idx, int64_t n, int64_t cap, void* ws, uint32_t num_tiles,
+    uint32_t tag) {
+  using XT = typename Bits<XB>::T;
+  using IT = typename Bits<IB>::T;
+  constexpr int G = kCmpG, J = kCmpJ, NW = kCmpNW;
+  constexpr int64_t GSPAN = (int64_t)kWave * G, WSEG = GSPAN * J;
+  __shared__ int s_wval[NW], s_wexcl[NW];
+  __shared__ int64_t s_base;
+
+  const int lane = __lane_id();
+  const int wid = threadIdx.x / kWave;
+  const uint32_t tile = blockIdx.x;
+  const int64_t t0 = (int64_t)tile * kCmpTile;
+  const int64_t wbase = t0 + (int64_t)wid * WSEG + (int64_t)lane * G;
+  const bool full = t0 + kCmpTile <= n;
+
+  int8_t m[J][G];
+  XT xv[J][G];
+  IT iv[J][G];
+#pragma unroll
+  for (int j = 0; j < J; ++j) {
+    const int64_t gb = wbase + j * GSPAN;
+    if (full) {
+      load_items<int8_t, G, true>(mask, gb, n, m[j]);
+      load_items<XT, G, true>(x, gb, n, xv[j]);
+      if constexpr (IDX == 1) load_items<IT, G, true>(idx_in, gb, n, iv[j]);
+    } else {
+      load_items<int8_t, G, false>(mask, gb, n, m[j]);
+      load_items<XT, G, false>(x, gb, n, xv[j]);
+      if constexpr (IDX == 1) load_items<IT, G, false>(idx_in, gb, n, iv[j]);
+    }
+  }
+
+  // Kept counts: per group, the lane's exclusive offset inside the wave.
+  int gex[J];
+  int wtot = 0;
+#pragma unroll
+  for (int j = 0; j < J; ++j) {
+    int c = 0;
+#pragma unroll
+    for (int i = 0; i < G; ++i) c += m[j][i] == 1;
+    const int incl = wave_incl_scan(c);
+    gex[j] = wtot + incl - c;
+    wtot += readlane(incl, kWave - 1);
+  }
+  if (lane == 0) s_wval[wid] = wtot;
+  __syncthreads();
+
+  if (wid == 0) {
+    int a = lane < NW ? s_wval[lane] : 0;
+    const int incl = wave_incl_scan(a);
+    const int agg = readlane(incl, NW - 1);
+    if (lane < NW) s_wexcl[lane] = incl - a;
+    auto fb = [&](int64_t u) {
+      const int64_t u0 = u * kCmpTile;
+      const int64_t u1 = u0 + kCmpTile < n ? u0 + kCmpTile : n;
+      int c = 0;
+      for (int64_t p = u0 + lane; p < u1; p += kWave) c += mask[p] == 1;
+      return FlagVal<int>{u == 0 ? 1 : 0, wave_sum(c)};
+    };
+    uint64_t* desc = ws_desc(ws);
+    int carry = 0;
+    if (tile == 0) {
+      if (lane == 0 && num_tiles > 1) desc_publish<int>(desc, 0, agg, tag, kFlagIncl);
+    } else {
+      if (lane == 0) desc_publish<int>(desc, tile, agg, tag, kFlagAgg);
+      carry = lookback<int>(desc, tile, tag, fb);
+      if (lane == 0) desc_publish<int>(desc, tile, carry + agg, tag, kFlagIncl);
+    }
+    // Counts are 32-bit: n < 2^31 per call (checked on the host).
+    if (lane == 0) s_base = carry;
+  }
+  __syncthreads();
+
+  const int64_t base = s_base + s_wexcl[wid];
+#pragma unroll
+  for (int j = 0; j < J; ++j) {
+    int64_t pos = base + gex[j];
+    const int64_t gb = wbase + j * GSPAN;
+#pragma unroll
+    for (int i = 0; i < G; ++i) {
+      if (m[j][i] == 1) {
+        if (pos < cap) {
+          out_x[pos] = xv[j][i];
+          if constexpr (IDX == 1) out_idx[pos] = iv[j][i];
+          if constexpr (IDX == 2) out_idx[pos] = (IT)(gb + i);
+          if constexpr (IDX == 3) out_idx[pos] = (IT)(gb + i + 1);
+        }
+        ++pos;
+      }
+    }
+  }
+}
+
+// Count of mask == 1: per-workgroup partial counts, then one workgroup sums
+// them in a fixed order.
+constexpr int kCountBlock = 256;
+constexpr int64_t kCountChunk = 1 << 16;
+
+__global__ __launch_bounds__(kCountBlock) void count_partials_kernel(const int8_t* __restrict__ mask,
+                                                                     int64_t n,
+                                                                     int64_t* __restrict__ partial) {
+  const int64_t c0 = (int64_t)blockIdx.x * kCountChunk;
+  const int64_t c1 = c0 + kCountChunk < n ? c0 + kCountChunk : n;
+  int c = 0;
+  const bool full = c1 - c0 == kCountChunk;
+  for (int64_t b = c0 + (int64_t)threadIdx.x * 16; b < c1; b += kCountBlock * 16) {
+    int8_t m[16];
+    if (full) {
+      load_items<int8_t, 16, true>(mask, b, c1, m);
+    } else {
+      load_items<int8_t, 16, false>(mask, b, c1, m);
+    }
+#pragma unroll
+    for (int i = 0; i < 16; ++i) c += m[i] == 1;
+  }
+  __shared__ int s[kCountBlock / kWave];
+  const int w = wave_sum(c);
+  if (__lane_id() == 0) s[threadIdx.x / kWave] = w;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int64_t t = 0;
+    for (int k = 0; k < kCountBlock / kWave; ++k) t += s[k];
+    partial[blockIdx.x] = t;
+  }
+}
+
+__global__ __launch_bounds__(kCountBlock) void count_final_kernel(const int64_t* __restrict__ partial,
+                                                                  int64_t np, int64_t* __restrict__ out) {
+  int64_t t = 0;
+  for (int64_t k = threadIdx.x; k < np; k += kCountBlock) t += partial[k];
+  __shared__ int64_t s[kCountBlock];
+  s[threadIdx.x] = t;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int64_t a = 0;
+    for (int k = 0; k < kCountBlock; ++k) a += s[k];
+    out[0] = a;
+  }
+}
+
+template <class T, int KIND>
+__device__ __forceinline__ T gather_mul(T v, T g);
+template <>
+__device__ __forceinline__ uint16_t gather_mul<uint16_t, 0>(uint16_t v, uint16_t g) {  // fp16
+  const float p = (float)__builtin_bit_cast(_Float16, v) * (float)__builtin_bit_cast(_Float16, g);
+  return __builtin_bit_cast(uint16_t, (_Float16)p);
+}
+template <>
+__device__ __forceinline__ uint16_t gather_mul<uint16_t, 1>(uint16_t v, uint16_t g) {  // int16
+  return (uint16_t)((int32_t)(int16_t)v * (int32_t)(int16_t)g);
+}
+template <>
+__device__ __forceinline__ uint32_t gather_mul<uint32_t, 2>(uint32_t v, uint32_t g) {  // fp32
+  return __float_as_uint(__uint_as_float(v) * __uint_as_float(g));
+}
+
+template <class T, int KIND>
+__global__ __launch_bounds__(256) void csr_gather_kernel(const T* __restrict__ vals,
+                                                         const int32_t* __restrict__ cols,
+                                                         const T* __restrict__ x, T* __restrict__ z,
+                                                         int64_t nnz, int64_t x_len) {
+  constexpr int G = 8;
+  const int64_t b = ((int64_t)blockIdx.x * 256 + threadIdx.x) * G;
+  if (b >= nnz) return;
+  T v[G], o[G];
+  int32_t c[G];
+  if (b + G <= nnz) {
+    load_items<T, G, true>(vals, b, nnz, v);
+    load_items<int32_t, G, true>(cols, b, nnz, c);
+  } else {
+    load_items<T, G, false>(vals, b, nnz, v);
+    load_items<int32_t, G, false>(cols, b, nnz, c);
+  }
+#pragma unroll
+  for (int i = 0; i < G; ++i) {
+    const T g = (uint32_t)c[i] < (uint64_t)x_len ? x[c[i]] : T(0);
+    o[i] = gather_mul<T, KIND>(v[i], g);
+  }
+  if (b + G <= nnz) {
+    store_items<T, G, true>(z, b, nnz, o);
+  } else {
+    store_items<T, G, false>(z, b, nnz, o);
+  }
+}
+
+template <int XB, int IB, int IDX>
+static int launch_compress(const void* x, const void* idx_in, const int8_t* mask, void* out_x,
+                           void* out_idx, int64_t n, int64_t cap, void* ws, uint32_t tiles,
+                           hipStream_t st) {
+  const uint32_t tag = next_launch_tag(ws);
+  hipLaunchKernelGGL((compress_kernel<XB, IB, IDX>), dim3(tiles), dim3(kCmpNW * kWave), 0, st,
+                     static_cast<const typename Bits<XB>::T*>(x),
+                     static_cast<const typename Bits<IB>::T*>(idx_in), mask,
+                     static_cast<typename Bits<XB>::T*>(out_x),
+                     static_cast<typename Bits<IB>::T*>(out_idx), n, cap, ws, tiles, tag);
+  TCS_CHECK_LAUNCH();
+  return TCS_OK;
+}
+
+template <int XB>
+static int compress_idx(int idx_mode, int idx_bytes, const void* x, const void* idx_in,
+                        const int8_t* mask, void* out_x, void* out_idx, int64_t n, int64_t cap,
+                        void* ws, uint32_t tiles, hipStream_t st) {
+  if (idx_mode == 0) return launch_compress<XB, 4, 0>(x, nullptr, mask, out_x, nullptr, n, cap, ws, tiles, st);
+  if (idx_mode == 2) return launch_compress<XB, 4, 2>(x, nullptr, mask, out_x, out_idx, n, cap, ws, tiles, st);
+  if (idx_mode == 3) return launch_compress<XB, 4, 3>(x, nullptr, mask, out_x, out_idx, n, cap, ws, tiles, st);
+  if (idx_bytes == 4) return launch_compress<XB, 4, 1>(x, idx_in, mask, out_x, out_idx, n, cap, ws, tiles, st);
+  if (idx_bytes == 8) return launch_compress<XB, 8, 1>(x, idx_in, mask, out_x, out_idx, n, cap, ws, tiles, st);
+  return TCS_ERR_UNSUPPORTED_DTYPE;
+}
+
+}  // namespace tcs
+
+extern "C" {
+
+size_t tcs_compress_workspace_bytes(int64_t n) {
+  if (n <= 0) return tcs::lookback_ws_bytes(0);
+  const uint64_t tiles = (uint64_t)((n + tcs::kCmpTile - 1) / tcs::kCmpTile);
+  const size_t lb = tcs::lookback_ws_bytes(tiles);
+  const size_t parts = tcs::kHeaderBytes + (size_t)((n + tcs::kCountChunk - 1) / tcs::kCountChunk) * 8;
+  return lb > parts ? lb : parts;
+}
+
+int tcs_count_mask(const int8_t* mask, int64_t n, int64_t* count_out, void* workspace,
+                   size_t workspace_bytes, void* stream) {
+  using namespace tcs;
+  if (n < 0 || !count_out || (n > 0 && !mask)) return TCS_ERR_INVALID_ARG;
+  if (n > 0 && !aligned16(mask)) return TCS_ERR_INVALID_ARG;
+  const int64_t np = (n + kCountChunk - 1) / kCountChunk;
+  if (!workspace || workspace_bytes < kHeaderBytes + (size_t)np * 8) return TCS_ERR_WORKSPACE;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  // Partials after the reserved header (the first bytes of a workspace hold
+  // the persistent scan's ticket word).
+  auto* partial = reinterpret_cast<int64_t*>(static_cast<char*>(workspace) + kHeaderBytes);
+  if (np > 0) {
+    hipLaunchKernelGGL(count_partials_kernel, dim3((uint32_t)np), dim3(kCountBlock), 0, st, mask, n, partial);
+    TCS_CHECK_LAUNCH();
+  }
+  hipLaunchKernelGGL(count_final_kernel, dim3(1), dim3(kCountBlock), 0, st, partial, np, count_out);
+  TCS_CHECK_LAUNCH();
+  return TCS_OK;
+}
+
+int tcs_compress(const void* x, int x_elem_bytes, const void* idx_in, int idx_mode, int idx_elem_bytes,
+                 const int8_t* mask, void* out_x, void* out_idx, int64_t n, int64_t out_capacity,
+                 void* workspace, size_t workspace_bytes, void* stream) {
+  using namespace tcs;
+  if (n < 0 || out_capacity < 0 || idx_mode < 0 || idx_mode > 3) return TCS_ERR_INVALID_ARG;
+  if (n == 0) return TCS_OK;
+  if (!x || !mask || (out_capacity > 0 && !out_x)) return TCS_ERR_INVALID_ARG;
+  if (idx_mode != 0 && out_capacity > 0 && !out_idx) return TCS_ERR_INVALID_ARG;
+  if (idx_mode == 1 && !idx_in) return TCS_ERR_INVALID_ARG;
+  if (idx_mode >= 2 && idx_elem_bytes != 4) return TCS_ERR_UNSUPPORTED_DTYPE;
+  if (!aligned16(x) || !aligned16(mask) || (idx_mode == 1 && !aligned16(idx_in))) return TCS_ERR_INVALID_ARG;
+  if (n >= ((int64_t)1 << 31)) return TCS_ERR_INVALID_ARG;
+  const uint64_t tiles = (uint64_t)((n + kCmpTile - 1) / kCmpTile);
+  if (!workspace || workspace_bytes < lookback_ws_bytes(tiles)) return TCS_ERR_WORKSPACE;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const uint32_t nt = (uint32_t)tiles;
+  switch (x_elem_bytes) {
+    case 1: return compress_idx<1>(idx_mode, idx_elem_bytes, x, idx_in, mask, out_x, out_idx, n, out_capacity, workspace, nt, st);
+    case 2: return compress_idx<2>(idx_mode, idx_elem_bytes, x, idx_in, mask, out_x, out_idx, n, out_capacity, workspace, nt, st);
+    case 4: return compress_idx<4>(idx_mode, idx_elem_bytes, x, idx_in, mask, out_x, out_idx, n, out_capacity, workspace, nt, st);
+    case 8: return compress_idx<8>(idx_mode, idx_elem_bytes, x, idx_in, mask, out_x, out_idx, n, out_capacity, workspace, nt, st);
+    default: return TCS_ERR_UNSUPPORTED_DTYPE;
+  }
+}
+
+int tcs_csr_gather(const void* vals, int dtype, const int32_t* cols, const void* x, int64_t x_len,
+                   void* z, int64_t nnz, void* stream) {
+  using namespace tcs;
+  if (nnz < 0 || x_len < 0) return TCS_ERR_INVALID_ARG;
+  if (nnz == 0) return TCS_OK;
+  if (!vals || !cols || !x || !z) return TCS_ERR_INVALID_ARG;
+  if (!aligned16(vals) || !aligned16(cols) || !aligned16(z)) return TCS_ERR_INVALID_ARG;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const uint32_t blocks = (uint32_t)((nnz + 256 * 8 - 1) / (256 * 8));
+  switch (dtype) {
+    case TCS_F16:
+      hipLaunchKernelGGL((csr_gather_kernel<uint16_t, 0>), dim3(blocks), dim3(256), 0, st,
+                         static_cast<const uint16_t*>(vals), cols, static_cast<const uint16_t*>(x),
+                         static_cast<uint16_t*>(z), nnz, x_len);
+      break;
+    case TCS_I16:
+      hipLaunchKernelGGL((csr_gather_kernel<uint16_t, 1>), dim3(blocks), dim3(256), 0, st,
+                         static_cast<const uint16_t*>(vals), cols, static_cast<const uint16_t*>(x),
+                         static_cast<uint16_t*>(z), nnz, x_len);
+      break;
+    case TCS_F32:
+      hipLaunchKernelGGL((csr_gather_kernel<uint32_t, 2>), dim3(blocks), dim3(256), 0, st,
+                         static_cast<const uint32_t*>(vals), cols, static_cast<const uint32_t*>(x),
+                         static_cast<uint32_t*>(z), nnz, x_len);
+      break;
+    default:
+      return TCS_ERR_UNSUPPORTED_DTYPE;
+  }
+  TCS_CHECK_LAUNCH();
+  return TCS_OK;
+}
+
+}  // extern "C"

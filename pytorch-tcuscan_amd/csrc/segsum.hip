@@ -1,0 +1,450 @@
+// Segmented sums over CSR row pointers and CSR SpMV on gfx950.
+//
+// y[r] = sum over row r of v[p], with row r covering [end(r-1), end(r)),
+// end(-1) = 0, end(r) = ends[r] (the reference passes indptr + 1 to its
+// kernels and treats row 0 as starting at 0: src/torch/torch_seg_ops.h:
+// 234-237, 378-381).  v[p] = x[p] for the segmented sums and
+// vals[p] * xv[cols[p]] for SpMV.
+//
+// Two launches.  The row pass gives every row one thread: empty rows get
+// y[r] = 0 and every tile records the row holding its first element.  The
+// nnz pass walks the nonzeros in tiles of NW * J * 64 * G elements with the
+// same lane layout as the scans: a tile stages the row ends that fall in it
+// in LDS, each lane locates its first row end by binary search and merges
+// the ends with its elements to find row heads and row ends, a segmented
+// scan runs over the tile, and the carry of a row that began in an earlier
+// tile arrives through the decoupled look-back.  Row sums are stored at the
+// row ends.  The work per tile is proportional to its nonzeros, however the
+// rows are distributed (lognormal rows of up to 20000 nonzeros are split
+// across tiles like any other).
+//
+// Accumulation: fp16/fp32 values in double (fp16 segment sums are exact and
+// correctly rounded to float; SpMV products are formed exactly), int8 in
+// int32 (exact).  The reference rounds SpMV products to fp16
+// (src/kernels/kernel_csr_gather.h:131-133); this build does not.
+#include "tcs_common.hpp"
+
+namespace tcs {
+
+constexpr int kSegNW = 8, kSegJ = 2, kSegG = 8;
+constexpr int64_t kSegTile = (int64_t)kSegNW * kSegJ * kWave * kSegG;  // 8192
+constexpr int kEndsCap = 4096;  // row ends staged in LDS per tile
+
+template <class IdxT>
+__device__ __forceinline__ int64_t row_end(const IdxT* __restrict__ ends, int64_t r, int64_t n_ends,
+                                           int64_t nnz) {
+  if (r < 0) return 0;
+  const int64_t e = r < n_ends ? (int64_t)ends[r] : nnz;
+  return e < nnz ? e : nnz;
+}
+
+// Row pass: zero empty rows, record each tile's first row.  tile_row[b] = R
+// marks a tile past the last row.
+template <class IdxT, class OutT>
+__global__ __launch_bounds__(256) void seg_rows_kernel(const IdxT* __restrict__ ends, int64_t R,
+                                                       int64_t n_ends, int64_t nnz,
+                                                       OutT* __restrict__ y,
+                                                       uint32_t* __restrict__ tile_row,
+                                                       uint32_t num_tiles) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= R) return;
+  const int64_t s = row_end(ends, r - 1, n_ends, nnz);
+  const int64_t e = row_end(ends, r, n_ends, nnz);
+  if (e <= s) {
+    y[r] = OutT(0);
+  } else {
+    for (int64_t b = (s + kSegTile - 1) / kSegTile; b * kSegTile < e && b < num_tiles; ++b)
+      tile_row[b] = (uint32_t)r;
+  }
+  if (r == R - 1) {
+    const int64_t last = e > s ? e : s;
+    for (int64_t b = (last + kSegTile - 1) / kSegTile; b < num_tiles; ++b) tile_row[b] = (uint32_t)R;
+  }
+}
+
+// Value sources.
+template <class C>
+struct SrcPlain {
+  using Acc = typename C::Acc;
+  using In = typename C::In;
+  const In* __restrict__ x;
+  struct Regs { In v[kSegJ][kSegG]; };
+  template <bool FULL>
+  __device__ __forceinline__ void load(Regs& r, int j, int64_t gb, int64_t t1) const {
+    load_items<In, kSegG, FULL>(x, gb, t1, r.v[j]);
+  }
+  __device__ __forceinline__ Acc value(const Regs& r, int j, int i) const { return C::conv(r.v[j][i]); }
+  __device__ __forceinline__ Acc value_at(int64_t p) const { return C::conv(x[p]); }
+  __device__ __forceinline__ void barrier(Regs& r) const {
+#pragma unroll
+    for (int j = 0; j < kSegJ; ++j) reg_barrier(r.v[j]);
+  }
+};
+
+// SpMV products vals[p] * xv[cols[p]] formed exactly in double.  Columns
+// outside [0, x_len) contribute 0.
+template <class C>
+struct SrcSpmv {
+  using Acc = double;
+  using In = typename C::In;
+  const In* __restrict__ vals;
+  const int32_t* __restrict__ cols;
+  const In* __restrict__ xv;
+  int64_t x_len;
+  struct Regs { In v[kSegJ][kSegG]; In g[kSegJ][kSegG]; };
+  template <bool FULL>
+  __device__ __forceinline__ void load(Regs& r, int j, int64_t gb, int64_t t1) const {
+    load_items<In, kSegG, FULL>(vals, gb, t1, r.v[j]);
+    int32_t c[kSegG];
+    load_items<int32_t, kSegG, FULL>(cols, gb, t1, c);
+#pragma unroll
+    for (int i = 0; i < kSegG; ++i)
+      r.g[j][i] = ((uint32_t)c[i] < (uint64_t)x_len && (FULL || gb + i < t1)) ? xv[c[i]] : In(0);
+  }
+  __device__ __forceinline__ Acc value(const Regs& r, int j, int i) const {
+    return (double)C::conv(r.v[j][i]) * (double)C::conv(r.g[j][i]);
+  }
+  __device__ __forceinline__ Acc value_at(int64_t p) const {
+    const int32_t c = cols[p];
+    if ((uint32_t)c >= (uint64_t)x_len) return 0.0;
+    return (double)C::conv(vals[p]) * (double)C::conv(xv[c]);
+  }
+  __device__ __forceinline__ void barrier(Regs& r) const {
+#pragma unroll
+    for (int j = 0; j < kSegJ; ++j) {
+      reg_barrier(r.v[j]);
+      reg_barrier(r.g[j]);
+    }
+  }
+};
+
+// First index k in [0, cnt) with list[k] >= key (cnt if none).
+__device__ __forceinline__ int lower_bound_u32(const uint32_t* list, int cnt, uint32_t key) {
+  int lo = 0, hi = cnt;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (list[mid] < key) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+template <class Src, class IdxT, class OutT>
+__global__ __launch_bounds__(kSegNW * kWave) void seg_reduce_kernel(
+    Src src, const IdxT* __restrict__ ends, int64_t R, int64_t n_ends, int64_t nnz,
+    OutT* __restrict__ y, const uint32_t* __restrict__ tile_row, uint32_t num_tiles, void* ws,
+    uint32_t tag) {
+  using Acc = typename Src::Acc;
+  constexpr int G = kSegG, J = kSegJ, NW = kSegNW;
+  constexpr int64_t GSPAN = (int64_t)kWave * G, WSEG = GSPAN * J;
+  __shared__ uint32_t s_list[kEndsCap];
+  __shared__ Acc s_wval[NW], s_wexcl[NW], s_carry;
+  __shared__ int s_wflag[NW], s_wexclf[NW];
+
+  const int lane = __lane_id();
+  const int wid = threadIdx.x / kWave;
+  const uint32_t tile = blockIdx.x;
+  const int64_t t0 = (int64_t)tile * kSegTile;
+  const int64_t end_all = row_end(ends, R - 1, n_ends, nnz);
+  if (t0 >= end_all) return;  // no row reaches this tile (nor any later one)
+  const int64_t t1 = t0 + kSegTile < end_all ? t0 + kSegTile : end_all;
+
+  // Rows touching the tile and the row ends around it: list[k] =
+  // end(r_lo - 1 + k).
+  int64_t r_lo = tile_row[tile];
+  int64_t r_hi = tile + 1 < num_tiles ? (int64_t)tile_row[tile + 1] : R - 1;
+  if (r_lo > R - 1) r_lo = R - 1;
+  if (r_hi > R - 1) r_hi = R - 1;
+  if (r_hi < r_lo) r_hi = r_lo;
+  const int64_t cnt64 = r_hi - r_lo + 2;
+  const bool in_lds = cnt64 <= kEndsCap;
+  const int cnt = in_lds ? (int)cnt64 : 0;
+  if (in_lds) {
+    for (int k = threadIdx.x; k < cnt; k += NW * kWave)
+      s_list[k] = (uint32_t)row_end(ends, r_lo - 1 + k, n_ends, nnz);
+  }
+  __syncthreads();
+
+  // Locates heads and ends in one lane group.  Returns the list index of
+  // the group's first relevant end, for the re-walk in the output pass.
+  auto list_at = [&](int64_t k) -> uint32_t {
+    return in_lds ? s_list[k] : (uint32_t)row_end(ends, r_lo - 1 + k, n_ends, nnz);
+  };
+  auto first_k = [&](int64_t gb) -> int64_t {
+    if (in_lds) return lower_bound_u32(s_list, cnt, (uint32_t)gb);
+    int64_t lo = 0, hi = cnt64;
+    while (lo < hi) {
+      const int64_t mid = (lo + hi) >> 1;
+      if (list_at(mid) < (uint64_t)gb) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+  };
+
+  typename Src::Regs regs;
+  const int64_t wbase = t0 + (int64_t)wid * WSEG + (int64_t)lane * G;
+  const bool full = t0 + kSegTile <= t1;
+  if (full) {
+#pragma unroll
+    for (int j = 0; j < J; ++j) src.template load<true>(regs, j, wbase + j * GSPAN, t1);
+  } else {
+#pragma unroll
+    for (int j = 0; j < J; ++j) src.template load<false>(regs, j, wbase + j * GSPAN, t1);
+  }
+
+  int64_t kfirst[J];
+  uint32_t hmask[J], emask[J];
+#pragma unroll
+  for (int j = 0; j < J; ++j) {
+    const int64_t gb = wbase + j * GSPAN;
+    int64_t k = first_k(gb);
+    kfirst[j] = k;
+    uint32_t h = 0, e = 0;
+    for (int i = 0; i < G; ++i) {
+      const int64_t p = gb + i;
+      if (p >= t1) break;
+      // A row ending exactly at p means the next non-empty row starts at p.
+      bool head = false;
+      while (k < cnt64 && list_at(k) <= (uint64_t)p) {
+        head |= list_at(k) == (uint64_t)p;
+        ++k;
+      }
+      if (head || p == 0) h |= 1u << i;
+      if (k < cnt64 && list_at(k) == (uint64_t)(p + 1)) e |= 1u << i;
+    }
+    hmask[j] = h;
+    emask[j] = e;
+  }
+
+  // Segmented reduction of the tile (as in the scans).
+  Acc gex[J];
+  int gexf[J];
+  Acc wtot = Acc(0);
+  int wtotf = 0;
+#pragma unroll
+  for (int j = 0; j < J; ++j) {
+    Acc tv = Acc(0);
+    int tf = 0;
+#pragma unroll
+    for (int i = 0; i < G; ++i) {
+      const Acc v = src.value(regs, j, i);
+      if (hmask[j] >> i & 1u) {
+        tv = v;
+        tf = 1;
+      } else {
+        tv = tv + v;
+      }
+    }
+    int wf = tf;
+    Acc wv = tv;
+    wave_seg_incl_scan(wf, wv);
+    const Acc ev = dpp_mov<kWaveShr1>(wv);
+    const int ef = dpp_i32<kWaveShr1>(wf);
+    gex[j] = ef ? ev : wtot + ev;
+    gexf[j] = wtotf | ef;
+    const Acc gt = readlane(wv, kWave - 1);
+    const int gf = readlane(wf, kWave - 1);
+    wtot = gf ? gt : wtot + gt;
+    wtotf |= gf;
+  }
+  if (lane == 0) {
+    s_wval[wid] = wtot;
+    s_wflag[wid] = wtotf;
+  }
+  __syncthreads();
+
+  if (wid == 0) {
+    Acc a = lane < NW ? s_wval[lane] : Acc(0);
+    int af = lane < NW ? s_wflag[lane] : 0;
+    wave_seg_incl_scan(af, a);
+    const Acc agg = readlane(a, NW - 1);
+    const int aggf = readlane(af, NW - 1);
+    const Acc xa = dpp_mov<kWaveShr1>(a);
+    const int xf = dpp_i32<kWaveShr1>(af);
+    if (lane < NW) {
+      s_wexcl[lane] = xa;
+      s_wexclf[lane] = xf;
+    }
+    const bool first_head = readlane((int)(hmask[0] & 1u), 0) != 0;
+    // Fallback: {row head inside tile u, sum since its last head}.
+    auto fb = [&](int64_t u) {
+      const int64_t u0 = u * kSegTile;
+      const int64_t u1 = u0 + kSegTile < end_all ? u0 + kSegTile : end_all;
+      // Row holding u1 - 1: first r with end(r) > u1 - 1.
+      int64_t lo = 0, hi = R - 1;
+      while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (row_end(ends, mid, n_ends, nnz) <= u1 - 1) lo = mid + 1; else hi = mid;
+      }
+      const int64_t s = row_end(ends, lo - 1, n_ends, nnz);
+      const int64_t from = s > u0 ? s : u0;
+      Acc sum = Acc(0);
+      for (int64_t p = from + lane; p < u1; p += kWave) sum = sum + src.value_at(p);
+      return FlagVal<Acc>{s >= u0 ? 1 : 0, wave_sum(sum)};
+    };
+    uint64_t* desc = ws_desc(ws);
+    Acc carry = Acc(0);
+    if (tile == 0) {
+      if (lane == 0 && num_tiles > 1) desc_publish<Acc>(desc, 0, agg, tag, kFlagIncl);
+    } else if (aggf) {
+      if (lane == 0) desc_publish<Acc>(desc, tile, agg, tag, kFlagIncl);
+      if (!first_head) carry = lookback<Acc>(desc, tile, tag, fb);
+    } else {
+      if (lane == 0) desc_publish<Acc>(desc, tile, agg, tag, kFlagAgg);
+      carry = lookback<Acc>(desc, tile, tag, fb);
+      if (lane == 0) desc_publish<Acc>(desc, tile, carry + agg, tag, kFlagIncl);
+    }
+    if (lane == 0) s_carry = carry;
+  }
+  __syncthreads();
+
+  src.barrier(regs);
+  Acc cin = s_carry;
+  {
+    const Acc wx = s_wexcl[wid];
+    cin = s_wexclf[wid] ? wx : cin + wx;
+  }
+#pragma unroll
+  for (int j = 0; j < J; ++j) {
+    if (!emask[j]) continue;
+    Acc run = gexf[j] ? gex[j] : cin + gex[j];
+    int64_t k = kfirst[j];
+    const int64_t gb = wbase + j * GSPAN;
+    for (int i = 0; i < G; ++i) {
+      const int64_t p = gb + i;
+      if (p >= t1) break;
+      const Acc v = src.value(regs, j, i);
+      if (hmask[j] >> i & 1u) {
+        run = v;
+      } else {
+        run = run + v;
+      }
+      if (emask[j] >> i & 1u) {
+        while (k + 1 < cnt64 && list_at(k) != (uint64_t)(p + 1)) ++k;
+        int64_t row = r_lo - 1 + k;
+        if (row < 0) row = 0;
+        if (row > R - 1) row = R - 1;
+        y[row] = (OutT)run;
+      }
+    }
+  }
+}
+
+struct SegSumPlan {
+  uint32_t num_tiles;
+  size_t ws_bytes;
+  size_t tile_row_off;
+};
+
+inline SegSumPlan seg_plan(int64_t nnz) {
+  SegSumPlan p;
+  p.num_tiles = (uint32_t)((nnz + kSegTile - 1) / kSegTile);
+  const size_t lb = lookback_ws_bytes(p.num_tiles);
+  p.tile_row_off = lb;
+  p.ws_bytes = lb + (((size_t)p.num_tiles * 4 + 255) & ~(size_t)255);
+  return p;
+}
+
+template <class Src, class IdxT, class OutT>
+static int launch_seg(const Src& src, const void* ends_v, int64_t R, int64_t n_ends, int64_t nnz,
+                      void* y_v, void* ws, size_t ws_bytes, hipStream_t st) {
+  const SegSumPlan plan = seg_plan(nnz);
+  if (!ws || ws_bytes < plan.ws_bytes) return TCS_ERR_WORKSPACE;
+  const auto* ends = static_cast<const IdxT*>(ends_v);
+  auto* y = static_cast<OutT*>(y_v);
+  auto* tile_row = reinterpret_cast<uint32_t*>(static_cast<char*>(ws) + plan.tile_row_off);
+  const int64_t row_blocks = (R + 255) / 256;
+  hipLaunchKernelGGL((seg_rows_kernel<IdxT, OutT>), dim3((uint32_t)row_blocks), dim3(256), 0, st,
+                     ends, R, n_ends, nnz, y, tile_row, plan.num_tiles);
+  TCS_CHECK_LAUNCH();
+  if (plan.num_tiles == 0) return TCS_OK;
+  const uint32_t tag = next_launch_tag(ws);
+  hipLaunchKernelGGL((seg_reduce_kernel<Src, IdxT, OutT>), dim3(plan.num_tiles), dim3(kSegNW * kWave),
+                     0, st, src, ends, R, n_ends, nnz, y, tile_row, plan.num_tiles, ws, tag);
+  TCS_CHECK_LAUNCH();
+  return TCS_OK;
+}
+
+template <class Src, class OutT>
+static int launch_seg_idx(const Src& src, const void* indptr, int idx_dtype, int64_t R,
+                          int64_t n_ends, int64_t nnz, void* y, void* ws, size_t ws_bytes,
+                          hipStream_t st) {
+  if (idx_dtype == TCS_I32)
+    return launch_seg<Src, int32_t, OutT>(src, indptr, R, n_ends, nnz, y, ws, ws_bytes, st);
+  if (idx_dtype == TCS_U32)
+    return launch_seg<Src, uint32_t, OutT>(src, indptr, R, n_ends, nnz, y, ws, ws_bytes, st);
+  if (idx_dtype == TCS_I64)
+    return launch_seg<Src, int64_t, OutT>(src, indptr, R, n_ends, nnz, y, ws, ws_bytes, st);
+  return TCS_ERR_UNSUPPORTED_DTYPE;
+}
+
+}  // namespace tcs
+
+extern "C" {
+
+size_t tcs_seg_sum_workspace_bytes(int64_t nnz) {
+  return tcs::seg_plan(nnz < 0 ? 0 : nnz).ws_bytes;
+}
+
+int tcs_seg_sum_indptr(const void* x, int x_dtype, const void* ends, int ends_dtype,
+                       int64_t num_segments, int64_t num_ends, void* y, int64_t nnz,
+                       void* workspace, size_t workspace_bytes, void* stream) {
+  using namespace tcs;
+  if (num_segments < 0 || nnz < 0 || num_ends < 0 || num_ends > num_segments)
+    return TCS_ERR_INVALID_ARG;
+  if (num_segments == 0) return TCS_OK;
+  if (!y || !ends || (nnz > 0 && !x)) return TCS_ERR_INVALID_ARG;
+  if ((nnz > 0 && !aligned16(x))) return TCS_ERR_INVALID_ARG;
+  if (nnz >= ((int64_t)1 << 32) || num_segments >= ((int64_t)1 << 32) - 1) return TCS_ERR_INVALID_ARG;
+  if (ends_dtype != TCS_I32 && ends_dtype != TCS_U32 && ends_dtype != TCS_I64)
+    return TCS_ERR_UNSUPPORTED_DTYPE;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  switch (x_dtype) {
+    case TCS_F16:
+      return launch_seg_idx<SrcPlain<CodecF16>, float>(SrcPlain<CodecF16>{static_cast<const uint16_t*>(x)},
+                                                       ends, ends_dtype, num_segments, num_ends, nnz, y,
+                                                       workspace, workspace_bytes, st);
+    case TCS_BF16:
+      return launch_seg_idx<SrcPlain<CodecBF16>, float>(SrcPlain<CodecBF16>{static_cast<const uint16_t*>(x)},
+                                                        ends, ends_dtype, num_segments, num_ends, nnz, y,
+                                                        workspace, workspace_bytes, st);
+    case TCS_F32:
+      return launch_seg_idx<SrcPlain<CodecF32>, float>(SrcPlain<CodecF32>{static_cast<const float*>(x)},
+                                                       ends, ends_dtype, num_segments, num_ends, nnz, y,
+                                                       workspace, workspace_bytes, st);
+    case TCS_I8:
+      return launch_seg_idx<SrcPlain<CodecI8>, int>(SrcPlain<CodecI8>{static_cast<const int8_t*>(x)},
+                                                    ends, ends_dtype, num_segments, num_ends, nnz, y,
+                                                    workspace, workspace_bytes, st);
+    default:
+      return TCS_ERR_UNSUPPORTED_DTYPE;
+  }
+}
+
+int tcs_spmv_csr(const void* vals, int vals_dtype, const void* indptr_ends, int ends_dtype,
+                 const int32_t* cols, const void* x, int64_t x_len, float* y, int64_t num_rows,
+                 int64_t nnz, void* workspace, size_t workspace_bytes, void* stream) {
+  using namespace tcs;
+  if (num_rows < 0 || nnz < 0 || x_len < 0) return TCS_ERR_INVALID_ARG;
+  if (num_rows == 0) return TCS_OK;
+  if (!y || !indptr_ends || (nnz > 0 && (!vals || !cols || !x))) return TCS_ERR_INVALID_ARG;
+  if (nnz > 0 && (!aligned16(vals) || !aligned16(cols))) return TCS_ERR_INVALID_ARG;
+  if (nnz >= ((int64_t)1 << 32) || num_rows >= ((int64_t)1 << 32) - 1) return TCS_ERR_INVALID_ARG;
+  if (ends_dtype != TCS_I32 && ends_dtype != TCS_U32 && ends_dtype != TCS_I64)
+    return TCS_ERR_UNSUPPORTED_DTYPE;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  switch (vals_dtype) {
+    case TCS_F16: {
+      SrcSpmv<CodecF16> src{static_cast<const uint16_t*>(vals), cols, static_cast<const uint16_t*>(x), x_len};
+      return launch_seg_idx<SrcSpmv<CodecF16>, float>(src, indptr_ends, ends_dtype, num_rows, num_rows, nnz,
+                                                      y, workspace, workspace_bytes, st);
+    }
+    case TCS_F32: {
+      SrcSpmv<CodecF32> src{static_cast<const float*>(vals), cols, static_cast<const float*>(x), x_len};
+      return launch_seg_idx<SrcSpmv<CodecF32>, float>(src, indptr_ends, ends_dtype, num_rows, num_rows, nnz,
+                                                      y, workspace, workspace_bytes, st);
+    }
+    default:
+      return TCS_ERR_UNSUPPORTED_DTYPE;
+  }
+}
+
+}  // extern "C"

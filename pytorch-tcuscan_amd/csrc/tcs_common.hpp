@@ -129,8 +129,11 @@ __device__ __forceinline__ void reg_barrier(T (&r)[ITEMS]) {
 }
 
 // ---------------------------------------------------------------------------
-// Workspace layout for look-back kernels: a 256-byte header whose first
-// word is the ticket/epoch word, then 16 bytes of descriptor per tile.
+// Workspace layout.  Every workspace starts with a 256-byte header whose
+// first word is the persistent scan's ticket counter; it is zero between
+// calls (each scan launch resets it) and no op puts scratch data there.
+// Look-back kernels keep 16 bytes of descriptor per tile right after the
+// header; other scratch follows the descriptors.
 // ---------------------------------------------------------------------------
 constexpr size_t kHeaderBytes = 256;
 constexpr size_t kDescBytes = 16;

@@ -57,6 +57,10 @@ template <int CTRL, int RM = 0xf, int BM = 0xf>
 __device__ __forceinline__ float dpp_mov(float v) { return dpp_f32<CTRL, RM, BM>(v); }
 template <int CTRL, int RM = 0xf, int BM = 0xf>
 __device__ __forceinline__ double dpp_mov(double v) { return dpp_f64<CTRL, RM, BM>(v); }
+template <int CTRL, int RM = 0xf, int BM = 0xf>
+__device__ __forceinline__ int64_t dpp_mov(int64_t v) {
+  return __builtin_bit_cast(int64_t, dpp_f64<CTRL, RM, BM>(__builtin_bit_cast(double, v)));
+}
 
 // DPP controls used below (GFX9 encoding).
 constexpr int kRowShr1 = 0x111, kRowShr2 = 0x112, kRowShr4 = 0x114, kRowShr8 = 0x118;
@@ -65,6 +69,10 @@ constexpr int kRowBcast15 = 0x142, kRowBcast31 = 0x143, kWaveShr1 = 0x138;
 __device__ __forceinline__ int readlane(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
 __device__ __forceinline__ float readlane(float v, int l) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ double readlane(double v, int l);
+__device__ __forceinline__ int64_t readlane(int64_t v, int l) {
+  return __builtin_bit_cast(int64_t, readlane(__builtin_bit_cast(double, v), l));
 }
 __device__ __forceinline__ double readlane(double v, int l) {
   const uint64_t b = __builtin_bit_cast(uint64_t, v);

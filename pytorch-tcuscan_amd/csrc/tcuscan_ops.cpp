@@ -293,6 +293,356 @@ at::Tensor run_scan_cpu(const at::Tensor& x) {
   return z;
 }
 
+// ---------------------------------------------------------------------------
+// Reduction / down-sweep building blocks (reference: src/torch/torch_reduce.h)
+// ---------------------------------------------------------------------------
+
+// src/torch/torch_reduce.h:42-75
+at::Tensor run_reduce_tiles(const at::Tensor& x_in, int64_t tile_len, int64_t num_blocks) {
+  const char* op = "run_reduce_tiles";
+  require_device(x_in, op, "x");
+  const auto t = x_in.scalar_type();
+  TORCH_CHECK(t == at::kHalf || t == at::kChar || t == at::kBFloat16 || t == at::kFloat || t == at::kInt,
+              op, ": x must be float16, bfloat16, float32, int8 or int32, got ", t);
+  TORCH_CHECK(tile_len > 0 && num_blocks > 0, op, ": tile_len and num_blocks must be positive");
+  at::Tensor x = prep(x_in);
+  at::Tensor z = at::empty({num_blocks}, x.options().dtype(scan_out_dtype(t)));
+  auto ws = workspace(x, tcs_reduce_tiles_workspace_bytes(x.numel(), tile_len, num_blocks));
+  const at::hip::HIPGuardMasqueradingAsCUDA guard(x.device());
+  check(tcs_reduce_tiles(x.data_ptr(), dtype_code(t), z.data_ptr(), x.numel(), tile_len, num_blocks,
+                         ws.first, ws.second, stream_of(x)),
+        op);
+  return z;
+}
+
+// src/torch/torch_reduce.h:143-182
+at::Tensor run_complete_rows(const at::Tensor& x_in, const at::Tensor& sums_in, int tile_width,
+                             int tile_height) {
+  const char* op = "run_complete_rows";
+  require_device(x_in, op, "x");
+  require_device(sums_in, op, "sums");
+  const auto t = x_in.scalar_type();
+  TORCH_CHECK(t == at::kFloat || t == at::kInt, op, ": x must be float32 or int32, got ", t);
+  TORCH_CHECK(sums_in.scalar_type() == t, op, ": sums must have the dtype of x");
+  TORCH_CHECK(tile_width > 0 && tile_height > 0, op, ": tile sizes must be positive");
+  TORCH_CHECK(sums_in.numel() > 0, op, ": sums must not be empty");
+  at::Tensor x = prep(x_in), sums = prep(sums_in);
+  at::Tensor z = at::empty({x.numel()}, x.options());
+  auto ws = workspace(x, tcs_complete_rows_workspace_bytes(x.numel(), sums.numel(), tile_width, tile_height));
+  const at::hip::HIPGuardMasqueradingAsCUDA guard(x.device());
+  check(tcs_complete_rows(x.data_ptr(), dtype_code(t), sums.data_ptr(), sums.numel(), z.data_ptr(),
+                          x.numel(), tile_width, tile_height, ws.first, ws.second, stream_of(x)),
+        op);
+  return z;
+}
+
+// src/torch/torch_reduce.h:197-231
+at::Tensor run_complete_blocks(const at::Tensor& x_in, const at::Tensor& sums_in, int tile_length) {
+  const char* op = "run_complete_blocks";
+  require_device(x_in, op, "x");
+  require_device(sums_in, op, "sums");
+  const auto t = x_in.scalar_type();
+  TORCH_CHECK(t == at::kFloat || t == at::kInt, op, ": x must be float32 or int32, got ", t);
+  TORCH_CHECK(sums_in.scalar_type() == t, op, ": sums must have the dtype of x");
+  TORCH_CHECK(tile_length > 0, op, ": tile_length must be positive");
+  TORCH_CHECK(sums_in.numel() > 0 && x_in.numel() % sums_in.numel() == 0, op,
+              ": len(x) must be a multiple of len(sums)");
+  at::Tensor x = prep(x_in), sums = prep(sums_in);
+  at::Tensor z = at::empty_like(x);
+  auto ws = workspace(x, tcs_complete_blocks_workspace_bytes(sums.numel()));
+  const at::hip::HIPGuardMasqueradingAsCUDA guard(x.device());
+  check(tcs_complete_blocks(x.data_ptr(), dtype_code(t), sums.data_ptr(), sums.numel(), z.data_ptr(),
+                            x.numel(), tile_length, ws.first, ws.second, stream_of(x)),
+        op);
+  return z;
+}
+
+// ---------------------------------------------------------------------------
+// Segmented sums and SpMV (reference: src/torch/torch_seg_ops.h, torch_spmv.h)
+// ---------------------------------------------------------------------------
+namespace {
+
+int index_code(const at::Tensor& t) {
+  switch (t.scalar_type()) {
+    case at::kInt: return TCS_I32;
+    case at::kUInt32: return TCS_U32;
+    case at::kLong: return TCS_I64;
+    default: return -1;
+  }
+}
+
+// Segment sums with row ends = ends_base[1 : 1 + num_ends] (the element
+// after the first), num_segments outputs.
+at::Tensor seg_sum_impl(const at::Tensor& x_in, const at::Tensor& idx_in, int64_t num_segments,
+                        int64_t num_ends, const char* op) {
+  require_device(x_in, op, "x");
+  require_device(idx_in, op, "indptr");
+  const auto t = x_in.scalar_type();
+  TORCH_CHECK(t == at::kHalf || t == at::kChar || t == at::kBFloat16 || t == at::kFloat, op,
+              ": x must be float16, bfloat16, float32 or int8, got ", t);
+  TORCH_CHECK(index_code(idx_in) >= 0, op, ": indptr must be int32, uint32 or int64, got ",
+              idx_in.scalar_type());
+  TORCH_CHECK(num_segments >= 0, op, ": indptr is too short");
+  at::Tensor x = prep(x_in);
+  at::Tensor idx = idx_in.contiguous();
+  at::Tensor z = at::empty({num_segments}, x.options().dtype(scan_out_dtype(t)));
+  if (num_segments == 0) return z;
+  auto ws = workspace(x, tcs_seg_sum_workspace_bytes(x.numel()));
+  const void* ends = static_cast<const char*>(idx.data_ptr()) + idx.element_size();
+  const at::hip::HIPGuardMasqueradingAsCUDA guard(x.device());
+  check(tcs_seg_sum_indptr(x.numel() ? x.data_ptr() : nullptr, dtype_code(t), ends, index_code(idx),
+                           num_segments, num_ends, z.data_ptr(), x.numel(), ws.first, ws.second,
+                           stream_of(x)),
+        op);
+  return z;
+}
+
+void check_segm_offsets(const c10::optional<at::Tensor>& so, const char* op) {
+  // The reference lets callers pass the per-core first segments of its
+  // fixed nnz partition (torch_seg_ops.h:362-375).  Work here is split by
+  // nonzeros inside the kernel, so the hint is accepted and not needed.
+  if (so.has_value() && so->defined()) {
+    TORCH_CHECK(so->dim() == 1, op, ": segm_offsets must be 1-D");
+  }
+}
+
+}  // namespace
+
+// src/torch/torch_seg_ops.h:211-266 (full CSR indptr, n + 1 entries)
+at::Tensor run_seg_sum_single_core(const at::Tensor& x, const at::Tensor& indptr, int s) {
+  TORCH_CHECK(s > 0, "run_seg_sum_single_core: s must be positive");
+  const int64_t R = indptr.numel() - 1;
+  return seg_sum_impl(x, indptr, R, R, "run_seg_sum_single_core");
+}
+
+// src/torch/torch_seg_ops.h:333-426
+at::Tensor run_seg_sum_multi_core(const at::Tensor& x, const at::Tensor& indptr, int s,
+                                  c10::optional<at::Tensor> segm_offsets) {
+  TORCH_CHECK(s > 0, "run_seg_sum_multi_core: s must be positive");
+  check_segm_offsets(segm_offsets, "run_seg_sum_multi_core");
+  const int64_t R = indptr.numel() - 1;
+  return seg_sum_impl(x, indptr, R, R, "run_seg_sum_multi_core");
+}
+
+// src/torch/torch_seg_ops.h:278-322: indptr holds the n segment starts
+// (indptr[:-1] of a CSR matrix); the last segment ends at len(x).
+at::Tensor run_seg_sum_single_cube(const at::Tensor& x, const at::Tensor& upper,
+                                   const at::Tensor& lower_strict, const at::Tensor& indptr, int s) {
+  const char* op = "run_seg_sum_single_cube";
+  TORCH_CHECK(s > 0, op, ": s must be positive");
+  TORCH_CHECK(upper.dim() == 2 && lower_strict.sizes() == upper.sizes(), op,
+              ": upper / lower_strict must be S x S");
+  const int64_t R = indptr.numel();
+  return seg_sum_impl(x, indptr, R, R > 0 ? R - 1 : 0, op);
+}
+
+// src/torch/torch_seg_ops.h:443-522 (fp16 only, full CSR indptr)
+at::Tensor run_seg_sum_multi_cube(const at::Tensor& x, const at::Tensor& upper,
+                                  const at::Tensor& lower_strict, const at::Tensor& indptr,
+                                  c10::optional<at::Tensor> segm_offsets) {
+  const char* op = "run_seg_sum_multi_cube";
+  TORCH_CHECK(x.scalar_type() == at::kHalf, op, ": x must be fp16, got ", x.scalar_type());
+  TORCH_CHECK(upper.dim() == 2 && lower_strict.sizes() == upper.sizes(), op,
+              ": upper / lower_strict must be S x S");
+  check_segm_offsets(segm_offsets, op);
+  const int64_t R = indptr.numel() - 1;
+  return seg_sum_impl(x, indptr, R, R, op);
+}
+
+// src/torch/torch_spmv.h:112-209
+at::Tensor run_spmv_v2(const at::Tensor& vals_in, const at::Tensor& indptr_in,
+                       const at::Tensor& cols_in, const at::Tensor& x_in, int s,
+                       c10::optional<at::Tensor> segm_offsets) {
+  const char* op = "run_spmv_v2";
+  const auto vt = vals_in.scalar_type(), xt = x_in.scalar_type();
+  TORCH_CHECK((vt == at::kHalf && xt == at::kHalf) || (vt == at::kFloat && xt == at::kFloat), op,
+              ": vals and x must both be fp16 or both be fp32, got vals=", vt, " x=", xt);
+  TORCH_CHECK(indptr_in.scalar_type() == at::kInt || indptr_in.scalar_type() == at::kUInt32, op,
+              ": indptr must be int32 or uint32, got ", indptr_in.scalar_type());
+  TORCH_CHECK(cols_in.scalar_type() == at::kInt || cols_in.scalar_type() == at::kUInt32, op,
+              ": cols must be int32 or uint32, got ", cols_in.scalar_type());
+  TORCH_CHECK(cols_in.numel() == vals_in.numel(), op, ": cols and vals must have the same length");
+  TORCH_CHECK(s > 0, op, ": s must be positive");
+  require_device(vals_in, op, "vals");
+  require_device(indptr_in, op, "indptr");
+  require_device(cols_in, op, "cols");
+  require_device(x_in, op, "x");
+  check_segm_offsets(segm_offsets, op);
+  at::Tensor vals = prep(vals_in), cols = prep(cols_in), x = x_in.contiguous();
+  at::Tensor indptr = indptr_in.contiguous();
+  const int64_t R = indptr.numel() - 1;
+  TORCH_CHECK(R >= 0, op, ": indptr must not be empty");
+  at::Tensor z = at::empty({R}, vals.options().dtype(at::kFloat));
+  if (R == 0) return z;
+  auto ws = workspace(vals, tcs_seg_sum_workspace_bytes(vals.numel()));
+  const void* ends = static_cast<const char*>(indptr.data_ptr()) + indptr.element_size();
+  const at::hip::HIPGuardMasqueradingAsCUDA guard(vals.device());
+  check(tcs_spmv_csr(vals.numel() ? vals.data_ptr() : nullptr, dtype_code(vt), ends, index_code(indptr),
+                     static_cast<const int32_t*>(cols.numel() ? cols.data_ptr() : nullptr),
+                     x.numel() ? x.data_ptr() : nullptr, x.numel(), z.data_ptr<float>(), R,
+                     vals.numel(), ws.first, ws.second, stream_of(vals)),
+        op);
+  return z;
+}
+
+// src/torch/torch_gather.h:92-172
+at::Tensor run_csr_gather(const at::Tensor& values_in, const at::Tensor& cols_in, const at::Tensor& x_in) {
+  const char* op = "run_csr_gather";
+  TORCH_CHECK(values_in.dim() == 1, op, ": values must be 1D, got ", values_in.dim(), "D");
+  TORCH_CHECK(cols_in.dim() == 1, op, ": cols must be 1D, got ", cols_in.dim(), "D");
+  TORCH_CHECK(x_in.dim() == 1, op, ": x must be 1D, got ", x_in.dim(), "D");
+  TORCH_CHECK(cols_in.numel() == values_in.numel(), op, ": cols length (", cols_in.numel(),
+              ") must match values length (", values_in.numel(), ")");
+  const auto t = values_in.scalar_type();
+  TORCH_CHECK(t == at::kHalf || t == at::kShort || t == at::kFloat, op,
+              ": values dtype must be fp16, int16, or fp32, got ", t);
+  TORCH_CHECK(x_in.scalar_type() == t, op, ": x dtype (", x_in.scalar_type(),
+              ") must match values dtype (", t, ")");
+  TORCH_CHECK(cols_in.scalar_type() == at::kInt, op, ": cols dtype must be int32, got ",
+              cols_in.scalar_type());
+  TORCH_CHECK(values_in.device() == cols_in.device() && values_in.device() == x_in.device(), op,
+              ": values, cols and x must be on the same device");
+  require_device(values_in, op, "values");
+  at::Tensor v = prep(values_in), c = prep(cols_in), x = x_in.contiguous();
+  at::Tensor z = at::empty_like(v);
+  const at::hip::HIPGuardMasqueradingAsCUDA guard(v.device());
+  check(tcs_csr_gather(v.data_ptr(), dtype_code(t), static_cast<const int32_t*>(c.data_ptr()),
+                       x.numel() ? x.data_ptr() : v.data_ptr(), x.numel(), z.data_ptr(), v.numel(),
+                       stream_of(v)),
+        op);
+  return z;
+}
+
+// ---------------------------------------------------------------------------
+// Compaction (reference: src/torch/torch_compress.h)
+// ---------------------------------------------------------------------------
+namespace {
+
+at::Tensor mask_i8(const at::Tensor& m, const char* op) {
+  TORCH_CHECK(m.scalar_type() == at::kChar || m.scalar_type() == at::kByte || m.scalar_type() == at::kBool,
+              op, ": mask must be int8 (uint8 and bool are accepted), got ", m.scalar_type());
+  return prep(m.scalar_type() == at::kChar ? m : m.to(at::kChar));
+}
+
+// Number of mask == 1 entries, brought to the host (the reference syncs
+// the same way to size its outputs, torch_compress.h:62-64).
+int64_t count_ones(const at::Tensor& mask, const char* op) {
+  at::Tensor cnt = at::empty({1}, mask.options().dtype(at::kLong));
+  auto ws = workspace(mask, tcs_compress_workspace_bytes(mask.numel()));
+  const at::hip::HIPGuardMasqueradingAsCUDA guard(mask.device());
+  check(tcs_count_mask(static_cast<const int8_t*>(mask.data_ptr()), mask.numel(), cnt.data_ptr<int64_t>(),
+                       ws.first, ws.second, stream_of(mask)),
+        op);
+  return cnt.item<int64_t>();
+}
+
+void compress_into(const at::Tensor& x, const at::Tensor* idx_in, int idx_mode, const at::Tensor& mask,
+                   at::Tensor& out_x, at::Tensor* out_idx, const char* op) {
+  const int64_t n = x.numel();
+  if (n == 0) return;
+  auto ws = workspace(x, tcs_compress_workspace_bytes(n));
+  const at::hip::HIPGuardMasqueradingAsCUDA guard(x.device());
+  check(tcs_compress(x.data_ptr(), (int)x.element_size(), idx_in ? idx_in->data_ptr() : nullptr, idx_mode,
+                     idx_in ? (int)idx_in->element_size() : 4, static_cast<const int8_t*>(mask.data_ptr()),
+                     out_x.numel() ? out_x.data_ptr() : nullptr,
+                     (out_idx && out_idx->numel()) ? out_idx->data_ptr() : nullptr, n, out_x.numel(),
+                     ws.first, ws.second, stream_of(x)),
+        op);
+}
+
+void compress_checks(const at::Tensor& x, const at::Tensor& mask, int S, const char* op) {
+  require_device(x, op, "x");
+  require_device(mask, op, "mask");
+  TORCH_CHECK(mask.numel() == x.numel(), op, ": x and mask must have the same length");
+  TORCH_CHECK(S > 0, op, ": S must be positive");
+  const auto es = x.element_size();
+  TORCH_CHECK(es == 1 || es == 2 || es == 4 || es == 8, op, ": unsupported element size");
+}
+
+}  // namespace
+
+// src/torch/torch_compress.h:41-98
+at::Tensor run_compress(const at::Tensor& x_in, const at::Tensor& mask_in, int S) {
+  const char* op = "run_compress";
+  compress_checks(x_in, mask_in, S, op);
+  at::Tensor x = prep(x_in), mask = mask_i8(mask_in, op);
+  const int64_t k = count_ones(mask, op);
+  at::Tensor z = at::empty({k}, x.options());
+  compress_into(x, nullptr, 0, mask, z, nullptr, op);
+  return z;
+}
+
+// src/torch/torch_compress.h:109-161: the output length is given.
+at::Tensor run_compress_pos(const at::Tensor& x_in, const at::Tensor& mask_in, int64_t output_len, int S) {
+  const char* op = "run_compress_pos";
+  compress_checks(x_in, mask_in, S, op);
+  TORCH_CHECK(output_len >= 0, op, ": output_len must be non-negative");
+  at::Tensor x = prep(x_in), mask = mask_i8(mask_in, op);
+  at::Tensor z = at::empty({output_len}, x.options());
+  compress_into(x, nullptr, 0, mask, z, nullptr, op);
+  return z;
+}
+
+// src/torch/torch_compress.h:172-239
+std::tuple<at::Tensor, at::Tensor> run_compress_ind(const at::Tensor& x_in, const at::Tensor& idx_in_t,
+                                                    const at::Tensor& mask_in, int S) {
+  const char* op = "run_compress_ind";
+  compress_checks(x_in, mask_in, S, op);
+  require_device(idx_in_t, op, "indices_in");
+  TORCH_CHECK(idx_in_t.numel() == x_in.numel(), op, ": indices_in and x must have the same length");
+  TORCH_CHECK(idx_in_t.element_size() == 4 || idx_in_t.element_size() == 8, op,
+              ": indices_in must have 4- or 8-byte elements");
+  at::Tensor x = prep(x_in), mask = mask_i8(mask_in, op), idx = prep(idx_in_t);
+  const int64_t k = count_ones(mask, op);
+  at::Tensor z = at::empty({k}, x.options());
+  at::Tensor zi = at::empty({k}, idx.options());
+  compress_into(x, &idx, 1, mask, z, &zi, op);
+  return std::make_tuple(z, zi);
+}
+
+// src/torch/torch_compress.h:249-309
+std::tuple<at::Tensor, at::Tensor> run_compress_ind_no_arange(const at::Tensor& x_in,
+                                                              const at::Tensor& mask_in, int S) {
+  const char* op = "run_compress_ind_no_arange";
+  compress_checks(x_in, mask_in, S, op);
+  at::Tensor x = prep(x_in), mask = mask_i8(mask_in, op);
+  const int64_t k = count_ones(mask, op);
+  at::Tensor z = at::empty({k}, x.options());
+  at::Tensor zi = at::empty({k}, x.options().dtype(at::kInt));
+  compress_into(x, nullptr, 2, mask, z, &zi, op);
+  return std::make_tuple(z, zi);
+}
+
+// src/torch/torch_seg_ops.h:85-107: f[i] = 1 marks the last element of a
+// segment; one float32 sum per flag.  Summed per segment directly (the
+// reference differences a global prefix, which loses precision).
+at::Tensor run_seg_sum(const at::Tensor& x_in, const at::Tensor& f_in, int S) {
+  const char* op = "run_seg_sum";
+  require_device(x_in, op, "x");
+  require_device(f_in, op, "f");
+  TORCH_CHECK(S > 0, op, ": S must be positive");
+  TORCH_CHECK(f_in.numel() == x_in.numel(), op, ": x and f must have the same length");
+  const auto t = x_in.scalar_type();
+  TORCH_CHECK(t == at::kHalf || t == at::kBFloat16 || t == at::kFloat || t == at::kChar, op,
+              ": x must be float16, bfloat16, float32 or int8, got ", t);
+  at::Tensor x = prep(x_in), f = mask_i8(f_in, op);
+  const int64_t k = count_ones(f, op);
+  at::Tensor ends = at::empty({k + 1}, x.options().dtype(at::kInt));
+  at::Tensor unused = at::empty({k}, f.options());
+  if (k > 0) {
+    at::Tensor ends_tail = ends.narrow(0, 1, k);
+    // ends[1 + j] = (position of the j-th flag) + 1.
+    auto ws = workspace(x, tcs_compress_workspace_bytes(f.numel()));
+    const at::hip::HIPGuardMasqueradingAsCUDA guard(x.device());
+    check(tcs_compress(f.data_ptr(), 1, nullptr, 3, 4, static_cast<const int8_t*>(f.data_ptr()),
+                       unused.data_ptr(), ends_tail.data_ptr(), f.numel(), k, ws.first, ws.second,
+                       stream_of(x)),
+          op);
+  }
+  at::Tensor sums = seg_sum_impl(x, ends, k, k, op);
+  return sums.scalar_type() == at::kFloat ? sums : sums.to(at::kFloat);
+}
+
 int abi_version() { return tcs_abi_version(); }
 
 }  // namespace tcuscan
@@ -313,4 +663,26 @@ PYBIND11_MODULE(tcuscan_ops, m) {
   m.def("run_seg_scan", &tcuscan::run_seg_scan, "Segmented Scan");
   m.def("run_seg_scan_vec", &tcuscan::run_seg_scan_vec, "Segmented Scan (vector-only)");
   m.def("run_scan_cpu", &tcuscan::run_scan_cpu, "Scan on CPUs");
+  m.def("run_reduce_tiles", &tcuscan::run_reduce_tiles, "Sum-reduce over tiles");
+  m.def("run_complete_rows", &tcuscan::run_complete_rows, "Down-sweep (second) phase of MCSCAN");
+  m.def("run_complete_blocks", &tcuscan::run_complete_blocks,
+        "Block-wise down-sweep (second) phase of block scan");
+  m.def("run_seg_sum", &tcuscan::run_seg_sum, "Segmented Sum");
+  m.def("run_seg_sum_single_core", &tcuscan::run_seg_sum_single_core, "Segmented Sum (single-core)");
+  m.def("run_seg_sum_multi_core", &tcuscan::run_seg_sum_multi_core, py::arg("x"), py::arg("indptr"),
+        py::arg("s"), py::arg("segm_offsets") = py::none(), "Segmented Sum (multi-core)");
+  m.def("run_seg_sum_single_cube", &tcuscan::run_seg_sum_single_cube, "Segmented Sum (single-cube)");
+  m.def("run_seg_sum_multi_cube", &tcuscan::run_seg_sum_multi_cube, py::arg("x"), py::arg("upper"),
+        py::arg("lower_strict"), py::arg("indptr"), py::arg("segm_offsets") = py::none(),
+        "Segmented Sum (multi-cube)");
+  m.def("run_spmv_v2", &tcuscan::run_spmv_v2, py::arg("vals"), py::arg("indptr"), py::arg("cols"),
+        py::arg("x"), py::arg("s"), py::arg("segm_offsets") = py::none(),
+        "Sparse Matrix-Vector Multiplication Using Segmented Sum");
+  m.def("run_csr_gather", &tcuscan::run_csr_gather, "CSR gather");
+  m.def("run_compress", &tcuscan::run_compress, "Compaction/compress");
+  m.def("run_compress_pos", &tcuscan::run_compress_pos,
+        "Compaction/compress with pre-computed output positions");
+  m.def("run_compress_ind", &tcuscan::run_compress_ind, "Compaction that returns indices");
+  m.def("run_compress_ind_no_arange", &tcuscan::run_compress_ind_no_arange,
+        "Compaction that returns indices (no input indices)");
 }
