@@ -12,9 +12,14 @@ Printed: one JSON line with the driver's fields plus
   cpu_baseline  the oracle's C port of the reference's run_scan_cpu
                 (three-phase OpenMP blocked scan), timed on this host over
                 whole copies of the same workload;
-  extras        the other large-scan workloads (int8 -> int32, bf16 -> fp32)
-                measured the same way.
-`--gpus N` with N > 1 exits: this build runs on one GPU.
+  extras        the other workloads measured the same way: the int8 and
+                bf16 scans, run_compress_ind on N = 2^28 fp32 values with a
+                Bernoulli(0.3) mask, and run_spmv_v2 on the 2^22 x 2^22 CSR
+                matrix with lognormal row lengths (mean 16, sigma 1.2,
+                clipped to [0, 20000]; about 6.7e7 nonzeros).
+`--only NAME` runs one workload for a fixed number of steps without the
+JSON line (for rocprofv3); `--gpus N` with N > 1 exits: this build runs on
+one GPU.
 """
 from __future__ import annotations
 
@@ -84,6 +89,83 @@ def _scan_workload(dtype_name: str, n: int, S: int, steps: int, warmup: int):
     }
 
 
+def _compress_workload(n: int, S: int, steps: int, warmup: int):
+    """run_compress_ind on N fp32 U[0,1) values, int8 Bernoulli(0.3) mask,
+    int32 arange indices.  Bytes: 4N + N + 4N read, 8K written."""
+    import torch
+
+    import tcuscan
+
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device="cpu").manual_seed(0)
+    x = torch.rand(n, generator=g).to(dev)
+    mask = (torch.rand(n, generator=g) < 0.3).to(torch.int8).to(dev)
+    idx = torch.arange(n, dtype=torch.int32, device=dev)
+    k = int(mask.sum().item())
+    torch.cuda.synchronize()
+    wall, ev_ms = _timed_steps(lambda: tcuscan.run_compress_ind(x, idx, mask, S), steps, warmup)
+    bytes_per_step = 9 * n + 8 * k
+    del x, mask, idx
+    torch.cuda.empty_cache()
+    return {"bytes_per_step": bytes_per_step, "wall_s": wall, "event_avg_ms": sum(ev_ms) / len(ev_ms),
+            "gbs_wall": bytes_per_step * steps / wall / 1e9, "kept": k}
+
+
+def spmv_matrix(rows: int, seed: int = 0):
+    """The SpMV workload's matrix on the GPU: rows = cols, lognormal row
+    lengths (mean 16, sigma 1.2, clipped to [0, 20000]), columns uniform and
+    sorted within each row, fp16 values and x uniform in [-1, 1]."""
+    import math
+
+    import numpy as np
+    import torch
+
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(seed)
+    mu = math.log(16.0) - 1.2 * 1.2 / 2.0
+    lens = np.clip(np.rint(rng.lognormal(mu, 1.2, size=rows)), 0, 20000).astype(np.int64)
+    indptr = torch.from_numpy(np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)).to(dev)
+    nnz = int(lens.sum())
+    g = torch.Generator(device=dev).manual_seed(seed)
+    row_id = torch.repeat_interleave(torch.arange(rows, device=dev), torch.from_numpy(lens).to(dev))
+    cols = torch.randint(0, rows, (nnz,), generator=g, device=dev)
+    key, _ = torch.sort(row_id * rows + cols)
+    cols = (key % rows).to(torch.int32)
+    del key, row_id
+    vals = (torch.rand(nnz, generator=g, device=dev) * 2 - 1).half()
+    x = (torch.rand(rows, generator=g, device=dev) * 2 - 1).half()
+    return vals, indptr, cols, x, nnz
+
+
+def _spmv_workload(rows: int, steps: int, warmup: int):
+    import torch
+
+    import tcuscan
+
+    vals, indptr, cols, x, nnz = spmv_matrix(rows)
+    torch.cuda.synchronize()
+    wall, ev_ms = _timed_steps(lambda: tcuscan.run_spmv_v2(vals, indptr, cols, x, 128), steps, warmup)
+    per = wall / steps
+    min_bytes = nnz * (2 + 4) + 4 * (rows + 1) + 2 * rows + 4 * rows
+    out = {"nnz": nnz, "wall_s": wall, "event_avg_ms": sum(ev_ms) / len(ev_ms),
+           "gflops": 2 * nnz / per / 1e9, "gbs_min_bytes": min_bytes / per / 1e9}
+    del vals, indptr, cols, x
+    torch.cuda.empty_cache()
+    return out
+
+
+def _copy_workload(n: int, steps: int, warmup: int):
+    """A plain float32 copy of n elements (PMC calibration: 4n bytes read
+    and 4n written per launch)."""
+    import torch
+
+    dev = torch.device("cuda:0")
+    a = torch.rand(n, device=dev)
+    b = torch.empty_like(a)
+    wall, ev_ms = _timed_steps(lambda: b.copy_(a), steps, warmup)
+    return {"gbs_wall": 8 * n * steps / wall / 1e9, "event_avg_ms": sum(ev_ms) / len(ev_ms)}
+
+
 def _cpu_baseline(n: int, budget_s: float = 10.0):
     """Times the oracle's C port of run_scan_cpu on the full fp16 workload."""
     import numpy as np
@@ -125,7 +207,9 @@ def main() -> int:
     ap.add_argument("--log2n", type=int, default=28)
     ap.add_argument("--S", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--no-extras", action="store_true", help="skip int8/bf16 extras")
+    ap.add_argument("--no-extras", action="store_true", help="skip the extra workloads")
+    ap.add_argument("--only", choices=["scan_f16", "scan_i8", "scan_bf16", "compress_ind", "spmv", "copy_f32"],
+                    help="run one workload and print its timing only (for profilers)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported", file=sys.stderr)
@@ -134,6 +218,18 @@ def main() -> int:
         return 0
 
     n = 1 << a.log2n
+    if a.only:
+        if a.only.startswith("scan_"):
+            r = _scan_workload({"scan_f16": "fp16", "scan_i8": "int8", "scan_bf16": "bf16"}[a.only], n, a.S,
+                               a.steps, a.warmup)
+        elif a.only == "compress_ind":
+            r = _compress_workload(n, a.S, a.steps, a.warmup)
+        elif a.only == "spmv":
+            r = _spmv_workload(1 << 22, a.steps, a.warmup)
+        else:
+            r = _copy_workload(n, a.steps, a.warmup)
+        print(json.dumps({"only": a.only, **{k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()}}))
+        return 0
     main_res = _scan_workload("fp16", n, a.S, a.steps, a.warmup)
     extras = {}
     if not a.no_extras:
@@ -145,6 +241,20 @@ def main() -> int:
                 "frac_of_peak": round(r["gbs_kernel"] / HBM_PEAK_GBS, 4),
                 "ms_per_step": round(r["wall_s"] * 1e3 / max(10, a.steps // 2), 4),
             }
+        ksteps = max(10, a.steps // 2)
+        c = _compress_workload(n, a.S, ksteps, max(3, a.warmup // 2))
+        extras[f"compress_ind_f32_2p{a.log2n}_p0.3"] = {
+            "value": round(c["gbs_wall"], 2), "unit": "GB/s",
+            "bytes_per_step": c["bytes_per_step"], "kept": c["kept"],
+            "ms_per_step": round(c["wall_s"] * 1e3 / ksteps, 4),
+            "note": "includes the device-to-host count read that sizes the outputs",
+        }
+        sp = _spmv_workload(1 << 22, ksteps, max(3, a.warmup // 2))
+        extras["spmv_v2_2p22_lognormal16"] = {
+            "value": round(sp["gflops"], 2), "unit": "GFLOP/s", "nnz": sp["nnz"],
+            "gbs_min_bytes": round(sp["gbs_min_bytes"], 2),
+            "ms_per_step": round(sp["wall_s"] * 1e3 / ksteps, 4),
+        }
     cpu = None if a.no_cpu else _cpu_baseline(n)
 
     ms_per_step = main_res["wall_s"] * 1e3 / a.steps
