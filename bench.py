@@ -57,7 +57,9 @@ def _timed_steps(fn, steps: int, warmup: int):
     return wall, [s.elapsed_time(e) for s, e in zip(starts, ends)]
 
 
-def _scan_workload(dtype_name: str, n: int, S: int, steps: int, warmup: int):
+def _scan_workload(dtype_name: str, n: int, S: int, steps: int, warmup: int, method: str | None = None):
+    """method None: run_scan_multi_core (the shipped local scan); "dpp" or
+    "mfma": run_scan_ex with that local scan method."""
     import torch
 
     import tcuscan
@@ -72,7 +74,11 @@ def _scan_workload(dtype_name: str, n: int, S: int, steps: int, warmup: int):
         x = (torch.rand(n, generator=g) * 2 - 1).to(dt).to(dev)
         in_b, out_b = 2, 4
     torch.cuda.synchronize()
-    wall, ev_ms = _timed_steps(lambda: tcuscan.run_scan_multi_core(x, S), steps, warmup)
+    if method is None:
+        fn = lambda: tcuscan.run_scan_multi_core(x, S)  # noqa: E731
+    else:
+        fn = lambda: tcuscan.run_scan_ex(x, S, 0, method)  # noqa: E731
+    wall, ev_ms = _timed_steps(fn, steps, warmup)
     ms = sorted(ev_ms)
     avg_ms = sum(ev_ms) / len(ev_ms)
     bytes_per_step = n * (in_b + out_b)
@@ -208,7 +214,8 @@ def main() -> int:
     ap.add_argument("--S", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-extras", action="store_true", help="skip the extra workloads")
-    ap.add_argument("--only", choices=["scan_f16", "scan_i8", "scan_bf16", "compress_ind", "spmv", "copy_f32"],
+    ap.add_argument("--only", choices=["scan_f16", "scan_i8", "scan_bf16", "scan_f16_mfma", "scan_i8_mfma",
+                                       "scan_f16_dpp", "scan_i8_dpp", "compress_ind", "spmv", "copy_f32"],
                     help="run one workload and print its timing only (for profilers)")
     a = ap.parse_args()
     if a.gpus != 1:
@@ -220,8 +227,9 @@ def main() -> int:
     n = 1 << a.log2n
     if a.only:
         if a.only.startswith("scan_"):
-            r = _scan_workload({"scan_f16": "fp16", "scan_i8": "int8", "scan_bf16": "bf16"}[a.only], n, a.S,
-                               a.steps, a.warmup)
+            parts = a.only.split("_")
+            r = _scan_workload({"f16": "fp16", "i8": "int8", "bf16": "bf16"}[parts[1]], n, a.S,
+                               a.steps, a.warmup, parts[2] if len(parts) > 2 else None)
         elif a.only == "compress_ind":
             r = _compress_workload(n, a.S, a.steps, a.warmup)
         elif a.only == "spmv":
@@ -233,9 +241,10 @@ def main() -> int:
     main_res = _scan_workload("fp16", n, a.S, a.steps, a.warmup)
     extras = {}
     if not a.no_extras:
-        for name in ("int8", "bf16"):
-            r = _scan_workload(name, n, a.S, max(10, a.steps // 2), max(3, a.warmup // 2))
-            extras[f"scan_{name}_2p{a.log2n}"] = {
+        for name, method in (("int8", None), ("bf16", None), ("fp16", "mfma"), ("int8", "mfma")):
+            r = _scan_workload(name, n, a.S, max(10, a.steps // 2), max(3, a.warmup // 2), method)
+            key = f"scan_{name}_2p{a.log2n}" + (f"_{method}" if method else "")
+            extras[key] = {
                 "value": round(r["gbs_wall"], 2), "unit": "GB/s",
                 "kernel_gbs": round(r["gbs_kernel"], 2),
                 "frac_of_peak": round(r["gbs_kernel"] / HBM_PEAK_GBS, 4),

@@ -72,7 +72,8 @@ size_t tcs_scan_workspace_bytes(int64_t n, int S, int tiles_per_wg);
  * result is the correctly rounded prefix whenever the exact prefix fits in
  * 53 bits); I8, I32 -> y int32 (exact, two's-complement wrap).
  * `start` is rounded to float (int32 for integer inputs) before use.
- * Single pass over HBM with decoupled look-back between tiles.
+ * Single pass over HBM with decoupled look-back between tiles; the local
+ * scan method is TCS_METHOD_AUTO (see tcs_scan_method).
  * Replaces: scan_multi_core_{fp16,int8}[_no_l2] (src/scan_multi_core.cpp:101,
  * 117,134,150) behind tcuscan::run_scan_multi_core / _no_l2
  * (src/torch/torch_scan.h:184,251); scan_single_core_{int8,fp16,fp32}
@@ -81,6 +82,27 @@ size_t tcs_scan_workspace_bytes(int64_t n, int S, int tiles_per_wg);
  * behind run_scan_multi_cube (src/torch/torch_scan.h:416). */
 int tcs_scan(const void* x, int x_dtype, void* y, int64_t n, int S, int tiles_per_wg,
              double start, void* workspace, size_t workspace_bytes, void* stream);
+
+/* Local scan method of tcs_scan_method. */
+enum tcs_scan_method {
+  TCS_METHOD_AUTO = 0, /* the shipped method, the faster one measured on MI355X:
+                          MFMA for plain int8 scans, DPP otherwise */
+  TCS_METHOD_DPP = 1,  /* lane-serial group sums joined by DPP wavefront scans */
+  TCS_METHOD_MFMA = 2  /* 32x32 units scanned on the matrix cores (F16, I8 only) */
+};
+
+/* tcs_scan with an explicit local scan method; results are identical for
+ * every method (bit-exact).  TCS_METHOD_MFMA multiplies each 1024-element
+ * unit, as 32 rows of 32, by a lower-triangular ones matrix on
+ * v_mfma_f32_32x32x16_f16 (fp16, split into two exactly-summed parts) or
+ * v_mfma_i32_32x32x32_i8 (int8), and is accepted for F16 and I8 inputs.
+ * Replaces: the Cube-unit row scan of scan_multi_core_{fp16,int8}
+ * (src/kernels/kernel_row_scan.h:97-131 via
+ * src/kernels/kernel_scan_multi_core.h:70-152) behind run_scan_multi_core
+ * (src/torch/torch_scan.h:184). */
+int tcs_scan_method(const void* x, int x_dtype, void* y, int64_t n, int S, int tiles_per_wg,
+                    double start, int method, void* workspace, size_t workspace_bytes,
+                    void* stream);
 
 /* Independent inclusive scans of consecutive rows of row_len elements
  * (the last row may be shorter).  Same dtypes as tcs_scan.
@@ -108,7 +130,7 @@ int tcs_seg_scan_flags(const void* x, int x_dtype, const int8_t* flags, void* y,
  * These keep the reference's block geometry.  reduce_tiles / complete_rows
  * split x into blocks [b*BL, min((b+1)*BL, n)) with BL = T * ceil(ceil(n/T)
  * / num_blocks) for a tile of T elements (scalar::GetWorkDistribution,
- * src/kernels/tcuscan_utils.h:1389-1408).  Every workspace's first 256 bytes
+ * src/kernels/tcuscan_utils.h:1389-1408).  Every workspace's first 2 KiB
  * are reserved (zero between calls); *_workspace_bytes include them.
  * --------------------------------------------------------------------- */
 

@@ -242,7 +242,7 @@ static RowsPlan rows_plan(int64_t n, int64_t num_sums, int64_t w, int64_t h) {
 
 size_t tcs_complete_rows_workspace_bytes(int64_t n, int64_t num_sums, int64_t tile_width,
                                          int64_t tile_height) {
-  if (n <= 0 || num_sums <= 0 || tile_width <= 0 || tile_height <= 0) return 256;
+  if (n <= 0 || num_sums <= 0 || tile_width <= 0 || tile_height <= 0) return tcs::kHeaderBytes;
   return rows_plan(n, num_sums, tile_width, tile_height).total;
 }
 

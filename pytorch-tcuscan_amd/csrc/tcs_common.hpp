@@ -129,13 +129,16 @@ __device__ __forceinline__ void reg_barrier(T (&r)[ITEMS]) {
 }
 
 // ---------------------------------------------------------------------------
-// Workspace layout.  Every workspace starts with a 256-byte header whose
-// first word is the persistent scan's ticket counter; it is zero between
-// calls (each scan launch resets it) and no op puts scratch data there.
+// Workspace layout.  Every workspace starts with a 2 KiB header holding the
+// persistent scan's kTicketCounters ticket counters, 256 bytes apart so that
+// their atomics do not serialise on one address; they are zero between
+// calls (each scan launch resets them) and no op puts scratch data there.
 // Look-back kernels keep 16 bytes of descriptor per tile right after the
 // header; other scratch follows the descriptors.
 // ---------------------------------------------------------------------------
-constexpr size_t kHeaderBytes = 256;
+constexpr int kTicketCounters = 8;
+constexpr size_t kTicketStride = 256;
+constexpr size_t kHeaderBytes = kTicketCounters * kTicketStride;
 constexpr size_t kDescBytes = 16;
 
 inline size_t lookback_ws_bytes(uint64_t num_tiles) {
@@ -143,7 +146,9 @@ inline size_t lookback_ws_bytes(uint64_t num_tiles) {
   return (b + 255) & ~(size_t)255;
 }
 
-__device__ __forceinline__ uint64_t* ws_header(void* ws) { return reinterpret_cast<uint64_t*>(ws); }
+__device__ __forceinline__ uint64_t* ws_ticket(void* ws, int k) {
+  return reinterpret_cast<uint64_t*>(reinterpret_cast<char*>(ws) + (size_t)k * kTicketStride);
+}
 __device__ __forceinline__ uint64_t* ws_desc(void* ws) {
   return reinterpret_cast<uint64_t*>(reinterpret_cast<char*>(ws) + kHeaderBytes);
 }

@@ -116,6 +116,18 @@ __device__ __forceinline__ void wave_seg_incl_scan(int& f, T& v) {
   seg_step<kRowBcast31, 0xc>(f, v);
 }
 
+// Inclusive sum scan inside each half wave (lanes 0-31 and 32-63
+// separately): the 64-lane scan without the row-1 -> rows-2/3 broadcast.
+template <class T>
+__device__ __forceinline__ T half_wave_incl_scan(T v) {
+  v = v + dpp_mov<kRowShr1>(v);
+  v = v + dpp_mov<kRowShr2>(v);
+  v = v + dpp_mov<kRowShr4>(v);
+  v = v + dpp_mov<kRowShr8>(v);
+  v = v + dpp_mov<kRowBcast15, 0xa>(v);
+  return v;
+}
+
 template <class T>
 __device__ __forceinline__ T wave_sum(T v) {
   return readlane(wave_incl_scan(v), kWave - 1);
@@ -124,7 +136,7 @@ __device__ __forceinline__ T wave_sum(T v) {
 // ---------------------------------------------------------------------------
 // Launch tags.
 //
-// Tiles are numbered by blockIdx.x.  Every launch on a workspace gets a tag
+// Every launch on a workspace gets a tag
 // in [1, 2^30 - 1] that differs from the tags of recent launches on that
 // workspace (the host keeps one counter per workspace), and descriptors carry
 // it, so descriptors left by earlier launches are never mistaken for current

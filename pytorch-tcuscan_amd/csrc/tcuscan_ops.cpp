@@ -145,6 +145,38 @@ at::Tensor run_scan_multi_core_no_l2(const at::Tensor& x, int S) {
   return multi_core_scan(x, S, "run_scan_multi_core_no_l2");
 }
 
+// Not in the reference: the plain scan with an explicit tile size
+// (tiles_per_wg S x S tiles, 0 = default) and local scan method ("auto",
+// "dpp" or "mfma"); every choice gives bit-identical results.
+at::Tensor run_scan_ex(const at::Tensor& x_in, int S, int64_t tiles_per_wg, const std::string& method,
+                       double starting_sum) {
+  const char* op = "run_scan_ex";
+  require_device(x_in, op, "x");
+  TORCH_CHECK(S > 0, op, ": S must be positive");
+  TORCH_CHECK(tiles_per_wg >= 0, op, ": tiles_per_wg must be non-negative");
+  int m = TCS_METHOD_AUTO;
+  if (method == "dpp") {
+    m = TCS_METHOD_DPP;
+  } else if (method == "mfma") {
+    m = TCS_METHOD_MFMA;
+  } else {
+    TORCH_CHECK(method == "auto", op, ": method must be 'auto', 'dpp' or 'mfma', got '", method, "'");
+  }
+  const auto t = x_in.scalar_type();
+  TORCH_CHECK(t == at::kHalf || t == at::kFloat || t == at::kChar || t == at::kBFloat16 || t == at::kInt,
+              op, ": x must be float16, bfloat16, float32, int8 or int32, got ", t);
+  at::Tensor x = prep(x_in);
+  at::Tensor z = at::empty({x.numel()}, x.options().dtype(scan_out_dtype(t)));
+  const int64_t n = x.numel();
+  const int tpw = (int)tiles_per_wg;
+  auto ws = workspace(x, tcs_scan_workspace_bytes(n, S, tpw));
+  const at::hip::HIPGuardMasqueradingAsCUDA guard(x.device());
+  check(tcs_scan_method(x.data_ptr(), dtype_code(t), z.data_ptr(), n, S, tpw, starting_sum, m, ws.first,
+                        ws.second, stream_of(x)),
+        op);
+  return z;
+}
+
 // src/torch/torch_scan.h:416-466: full scan; S is upper.size(0).
 at::Tensor run_scan_multi_cube(const at::Tensor& x_in, const at::Tensor& upper,
                                const at::Tensor& lower_strict) {
@@ -660,6 +692,9 @@ PYBIND11_MODULE(tcuscan_ops, m) {
   m.def("run_row_scan", &tcuscan::run_row_scan, "Row scan of each S-row");
   m.def("run_block_scan", &tcuscan::run_block_scan, "Block scan on blocks of length S^2");
   m.def("run_scan_multi_cube", &tcuscan::run_scan_multi_cube, "Multi-cube scan");
+  m.def("run_scan_ex", &tcuscan::run_scan_ex, py::arg("x"), py::arg("S"), py::arg("tiles_per_wg") = 0,
+        py::arg("method") = "auto", py::arg("starting_sum") = 0.0,
+        "Scan with an explicit tile size and local scan method ('auto', 'dpp', 'mfma')");
   m.def("run_seg_scan", &tcuscan::run_seg_scan, "Segmented Scan");
   m.def("run_seg_scan_vec", &tcuscan::run_seg_scan_vec, "Segmented Scan (vector-only)");
   m.def("run_scan_cpu", &tcuscan::run_scan_cpu, "Scan on CPUs");
