@@ -172,6 +172,73 @@ def _copy_workload(n: int, steps: int, warmup: int):
     return {"gbs_wall": 8 * n * steps / wall / 1e9, "event_avg_ms": sum(ev_ms) / len(ev_ms)}
 
 
+def _batched_workload(steps: int, warmup: int, num_scans: int = 10_000, num_segsums: int = 2_000):
+    """The batched uneven workload: `num_scans` fp16 scans with lengths
+    log-uniform in [2^8, 2^20], concatenated with int8 start flags, plus
+    `num_segsums` indptr seg-sums over int8 data (geometric segment lengths,
+    mean 64; 2^U(4, 12) segments per problem).  The batched form is one
+    BatchPlan (tuned, then captured as one HIP graph) launched per step; the
+    per-call form launches run_scan_multi_core on every scan and
+    run_seg_sum_multi_core on every seg-sum problem, each its own tensor."""
+    import numpy as np
+    import torch
+
+    import tcuscan
+    from tcuscan import workloads
+
+    dev = torch.device("cuda:0")
+    sc = workloads.uneven_scans(num_scans, 8, 20, seed=0, device=dev)
+    ss = workloads.geometric_segsums(num_segsums, 64.0, 4, 12, seed=1, device=dev)
+    plan = tcuscan.BatchPlan()
+    y_scan = plan.add_seg_scan(sc.x, sc.flags)
+    y_sum = plan.add_seg_sum(ss.x, ss.indptr)
+    t0 = time.perf_counter()
+    plan.tune(reps=2)
+    tune_s = time.perf_counter() - t0
+    plan.instantiate(max_branches=2)
+    wall_b, ev_b = _timed_steps(plan.launch, steps, warmup)
+
+    scan_pieces = sc.pieces()
+    sum_pieces = ss.pieces()
+
+    def per_call():
+        for p in scan_pieces:
+            tcuscan.run_scan_multi_core(p, 128)
+        for xi, ipi in sum_pieces:
+            tcuscan.run_seg_sum_multi_core(xi, ipi, 128)
+
+    pc_steps = max(2, min(steps, 5))
+    wall_p, _ = _timed_steps(per_call, pc_steps, 1)
+    # Same results both ways (spot check of 50 scans and 50 seg-sums).
+    rng = np.random.default_rng(0)
+    same = True
+    for i in rng.choice(num_scans, 50, replace=False):
+        o0, o1 = sc.offsets[i], sc.offsets[i + 1]
+        same &= bool(torch.equal(tcuscan.run_scan_multi_core(scan_pieces[i], 128), y_scan[o0:o1]))
+    for i in rng.choice(num_segsums, 50, replace=False):
+        r0, r1 = ss.row_off[i], ss.row_off[i + 1]
+        same &= bool(torch.equal(tcuscan.run_seg_sum_multi_core(*sum_pieces[i], 128), y_sum[r0:r1]))
+    total = int(sc.offsets[-1])
+    nnz = int(ss.indptr_host[-1])
+    rows = int(ss.row_off[-1])
+    bytes_per_step = 7 * total + nnz + 4 * (rows + 1) + 4 * rows
+    cfg = plan.configs()
+    del sc, ss, plan, scan_pieces, sum_pieces, y_scan, y_sum
+    torch.cuda.empty_cache()
+    ms_b = wall_b * 1e3 / steps
+    ms_p = wall_p * 1e3 / pc_steps
+    return {
+        "scans": num_scans, "scan_elems": total, "segsums": num_segsums, "segsum_nnz": nnz,
+        "segsum_rows": rows, "bytes_per_step": bytes_per_step,
+        "batched_ms": ms_b, "batched_event_avg_ms": sum(ev_b) / len(ev_b),
+        "per_call_ms": ms_p, "speedup": ms_p / ms_b,
+        "gbs_batched": bytes_per_step / (ms_b * 1e-3) / 1e9,
+        "tuned": {"seg_scan": {"S": cfg[0][0], "tiles_per_wg": cfg[0][1], "method": cfg[0][2],
+                               "window": cfg[0][3], "ms": round(cfg[0][4], 4)}},
+        "tune_s": tune_s, "same_results": same,
+    }
+
+
 def _cpu_baseline(n: int, budget_s: float = 10.0):
     """Times the oracle's C port of run_scan_cpu on the full fp16 workload."""
     import numpy as np
@@ -215,7 +282,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-extras", action="store_true", help="skip the extra workloads")
     ap.add_argument("--only", choices=["scan_f16", "scan_i8", "scan_bf16", "scan_f16_mfma", "scan_i8_mfma",
-                                       "scan_f16_dpp", "scan_i8_dpp", "compress_ind", "spmv", "copy_f32"],
+                                       "scan_f16_dpp", "scan_i8_dpp", "compress_ind", "spmv", "copy_f32",
+                                       "batched"],
                     help="run one workload and print its timing only (for profilers)")
     a = ap.parse_args()
     if a.gpus != 1:
@@ -234,6 +302,8 @@ def main() -> int:
             r = _compress_workload(n, a.S, a.steps, a.warmup)
         elif a.only == "spmv":
             r = _spmv_workload(1 << 22, a.steps, a.warmup)
+        elif a.only == "batched":
+            r = _batched_workload(a.steps, a.warmup)
         else:
             r = _copy_workload(n, a.steps, a.warmup)
         print(json.dumps({"only": a.only, **{k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()}}))
@@ -263,6 +333,13 @@ def main() -> int:
             "value": round(sp["gflops"], 2), "unit": "GFLOP/s", "nnz": sp["nnz"],
             "gbs_min_bytes": round(sp["gbs_min_bytes"], 2),
             "ms_per_step": round(sp["wall_s"] * 1e3 / ksteps, 4),
+        }
+        bt = _batched_workload(max(5, a.steps // 5), 2)
+        extras["batched_uneven_10k_scans_2k_segsums"] = {
+            "value": round(bt["speedup"], 2), "unit": "x vs per-call launches",
+            "batched_ms": round(bt["batched_ms"], 3), "per_call_ms": round(bt["per_call_ms"], 2),
+            "gbs_batched": round(bt["gbs_batched"], 1), "scan_elems": bt["scan_elems"],
+            "segsum_nnz": bt["segsum_nnz"], "tuned": bt["tuned"], "same_results": bt["same_results"],
         }
     cpu = None if a.no_cpu else _cpu_baseline(n)
 

@@ -91,8 +91,9 @@ enum tcs_scan_method {
   TCS_METHOD_MFMA = 2  /* 32x32 units scanned on the matrix cores (F16, I8 only) */
 };
 
-/* tcs_scan with an explicit local scan method; results are identical for
- * every method (bit-exact).  TCS_METHOD_MFMA multiplies each 1024-element
+/* tcs_scan with an explicit local scan method and look-back window
+ * (predecessor tiles polled per step, 1..64; 0 = default 16); results are
+ * identical for every choice (bit-exact).  TCS_METHOD_MFMA multiplies each 1024-element
  * unit, as 32 rows of 32, by a lower-triangular ones matrix on
  * v_mfma_f32_32x32x16_f16 (fp16, split into two exactly-summed parts) or
  * v_mfma_i32_32x32x32_i8 (int8), and is accepted for F16 and I8 inputs.
@@ -101,8 +102,8 @@ enum tcs_scan_method {
  * src/kernels/kernel_scan_multi_core.h:70-152) behind run_scan_multi_core
  * (src/torch/torch_scan.h:184). */
 int tcs_scan_method(const void* x, int x_dtype, void* y, int64_t n, int S, int tiles_per_wg,
-                    double start, int method, void* workspace, size_t workspace_bytes,
-                    void* stream);
+                    double start, int method, int lookback_window, void* workspace,
+                    size_t workspace_bytes, void* stream);
 
 /* Independent inclusive scans of consecutive rows of row_len elements
  * (the last row may be shorter).  Same dtypes as tcs_scan.
@@ -234,6 +235,67 @@ int tcs_count_mask(const int8_t* mask, int64_t n, int64_t* count_out, void* work
 int tcs_compress(const void* x, int x_elem_bytes, const void* idx_in, int idx_mode, int idx_elem_bytes,
                  const int8_t* mask, void* out_x, void* out_idx, int64_t n, int64_t out_capacity,
                  void* workspace, size_t workspace_bytes, void* stream);
+
+/* ------------------------------------------------------------------------
+ * Batched work plans
+ *
+ * No counterpart in the reference, whose wrappers launch and synchronise
+ * per call (e.g. src/torch/torch_scan.h:107) and fix their tiling by host
+ * rules (the src/tiling headers).  A plan records ops over fixed device buffers,
+ * tunes them and replays them all as one HIP graph launch.  Buffers must
+ * keep their addresses while the plan is used.  Each op owns a region of
+ * the plan's workspace (zeroed once, like every workspace), so ops run on
+ * parallel graph branches without sharing look-back state.  A plan is used
+ * by one thread and one stream at a time.
+ * --------------------------------------------------------------------- */
+
+typedef struct tcs_batch tcs_batch;
+
+int tcs_batch_create(tcs_batch** out);
+void tcs_batch_destroy(tcs_batch* b);
+int tcs_batch_num_ops(const tcs_batch* b);
+
+/* Each add returns the op index (>= 0) or a negated tcs_status.  Semantics
+ * are those of tcs_scan (start 0), tcs_seg_scan_flags, tcs_seg_sum_indptr and
+ * tcs_spmv_csr.  Adding an op drops a captured graph. */
+int tcs_batch_add_scan(tcs_batch* b, const void* x, int x_dtype, void* y, int64_t n);
+int tcs_batch_add_seg_scan(tcs_batch* b, const void* x, int x_dtype, const int8_t* flags, void* y,
+                           int64_t n);
+int tcs_batch_add_seg_sum(tcs_batch* b, const void* x, int x_dtype, const void* ends, int ends_dtype,
+                          int64_t num_segments, int64_t num_ends, void* y, int64_t nnz);
+int tcs_batch_add_spmv(tcs_batch* b, const void* vals, int vals_dtype, const void* indptr_ends,
+                       int ends_dtype, const int32_t* cols, const void* x, int64_t x_len, float* y,
+                       int64_t num_rows, int64_t nnz);
+
+/* Workspace bytes of the whole plan (every op's region). */
+size_t tcs_batch_workspace_bytes(const tcs_batch* b);
+
+/* Autotuner.  For every scan op, times each candidate tiling -- tiles of
+ * S*S*tiles_per_wg elements for S in {32, 64, 128} and tiles_per_wg in
+ * {1, ..., 32} (distinct tiles only), the local scan method (DPP, and MFMA
+ * for plain F16/I8 scans) and look-back windows {8, 16, 32} -- on the op's
+ * own buffers with HIP events (one warm-up, then `reps` timed runs), and
+ * keeps the fastest.  Results are cached per (op kind, dtype, ceil(log2 n))
+ * for the process; `force` re-tunes cached keys.  Synchronises `stream`. */
+int tcs_batch_tune(tcs_batch* b, void* workspace, size_t workspace_bytes, int reps, int force,
+                   void* stream);
+
+/* Configuration of op `op` (S, tiles_per_wg, method, window; ms = the
+ * tuner's best time, -1 if untuned). */
+int tcs_batch_op_config(const tcs_batch* b, int op, int* S, int* tiles_per_wg, int* method,
+                        int* window, float* ms);
+
+/* Runs every op in order on `stream` without a graph. */
+int tcs_batch_run_eager(tcs_batch* b, void* workspace, size_t workspace_bytes, void* stream);
+
+/* Captures every op into one HIP graph (on a stream of the plan's own;
+ * `stream` is synchronised first): ops are dealt round-robin to up to
+ * max_branches parallel branches forked from and joined to the origin. */
+int tcs_batch_instantiate(tcs_batch* b, int max_branches, void* workspace, size_t workspace_bytes,
+                          void* stream);
+
+/* One launch of the captured graph on `stream`. */
+int tcs_batch_launch(tcs_batch* b, void* stream);
 
 #ifdef __cplusplus
 }

@@ -35,15 +35,16 @@ template <int XB, int IB, int IDX>
 __global__ __launch_bounds__(kCmpNW * kWave) void compress_kernel(
     const typename Bits<XB>::T* __restrict__ x, const typename Bits<IB>::T* __restrict__ idx_in,
     const int8_t* __restrict__ mask, typename Bits<XB>::T* __restrict__ out_x,
-    typename Bits<IB>::T* __restrict__ out_idx, int64_t n, int64_t cap, void* ws, uint32_t num_tiles,
-    uint32_t tag) {
+    typename Bits<IB>::T* __restrict__ out_idx, int64_t n, int64_t cap, void* ws, uint32_t num_tiles) {
   using XT = typename Bits<XB>::T;
   using IT = typename Bits<IB>::T;
   constexpr int G = kCmpG, J = kCmpJ, NW = kCmpNW;
   constexpr int64_t GSPAN = (int64_t)kWave * G, WSEG = GSPAN * J;
   __shared__ int s_wval[NW], s_wexcl[NW];
   __shared__ int64_t s_base;
+  __shared__ uint32_t s_tag;
 
+  if (threadIdx.x == 0) s_tag = launch_arrive(ws);
   const int lane = __lane_id();
   const int wid = threadIdx.x / kWave;
   const uint32_t tile = blockIdx.x;
@@ -96,6 +97,7 @@ __global__ __launch_bounds__(kCmpNW * kWave) void compress_kernel(
       return FlagVal<int>{u == 0 ? 1 : 0, wave_sum(c)};
     };
     uint64_t* desc = ws_desc(ws);
+    const uint32_t tag = s_tag;
     int carry = 0;
     if (tile == 0) {
       if (lane == 0 && num_tiles > 1) desc_publish<int>(desc, 0, agg, tag, kFlagIncl);
@@ -225,12 +227,11 @@ template <int XB, int IB, int IDX>
 static int launch_compress(const void* x, const void* idx_in, const int8_t* mask, void* out_x,
                            void* out_idx, int64_t n, int64_t cap, void* ws, uint32_t tiles,
                            hipStream_t st) {
-  const uint32_t tag = next_launch_tag(ws);
   hipLaunchKernelGGL((compress_kernel<XB, IB, IDX>), dim3(tiles), dim3(kCmpNW * kWave), 0, st,
                      static_cast<const typename Bits<XB>::T*>(x),
                      static_cast<const typename Bits<IB>::T*>(idx_in), mask,
                      static_cast<typename Bits<XB>::T*>(out_x),
-                     static_cast<typename Bits<IB>::T*>(out_idx), n, cap, ws, tiles, tag);
+                     static_cast<typename Bits<IB>::T*>(out_idx), n, cap, ws, tiles);
   TCS_CHECK_LAUNCH();
   return TCS_OK;
 }

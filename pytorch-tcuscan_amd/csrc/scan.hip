@@ -20,9 +20,10 @@ int tcs_scan(const void* x, int x_dtype, void* y, int64_t n, int S, int tiles_pe
 }
 
 int tcs_scan_method(const void* x, int x_dtype, void* y, int64_t n, int S, int tiles_per_wg,
-                    double start, int method, void* workspace, size_t workspace_bytes, void* stream) {
+                    double start, int method, int lookback_window, void* workspace,
+                    size_t workspace_bytes, void* stream) {
   return tcs::scan_dispatch<tcs::kSegNone>(x, x_dtype, y, nullptr, n, 0, S, tiles_per_wg, start,
-                                           workspace, workspace_bytes, stream, method);
+                                           workspace, workspace_bytes, stream, method, lookback_window);
 }
 
 }  // extern "C"

@@ -140,7 +140,8 @@ struct ScanArgs {
   int64_t n;
   int64_t seg_len;
   uint32_t num_tiles;
-  uint32_t tag;
+  uint32_t tag;     // set in the kernel from the workspace epoch
+  int window;       // look-back window (predecessors polled per step)
 };
 
 // Element offset, within a data wave's part of the tile, of the G elements
@@ -284,9 +285,9 @@ __device__ __forceinline__ void tile_lookback(PipeSlot<C, NW>& sl,
   if (tile == 0) {
     if constexpr (!kSeg) carry = start;
   } else if (sl.aggf) {
-    if (!sl.first_head) carry = lookback<Acc>(desc, tile, a.tag, fb);
+    if (!sl.first_head) carry = lookback<Acc>(desc, tile, a.tag, fb, a.window);
   } else {
-    carry = lookback<Acc>(desc, tile, a.tag, fb);
+    carry = lookback<Acc>(desc, tile, a.tag, fb, a.window);
     if (lane == 0) desc_publish<Acc>(desc, tile, carry + sl.agg, a.tag, kFlagIncl);
   }
   if (lane == 0) sl.carry = carry;
@@ -425,6 +426,7 @@ __global__ __launch_bounds__((NW + 1) * kWave) void scan_kernel(
   constexpr int kStage = LOC == kLocMfma ? UnitScan<C>::kStageHalfs : 1;
   __shared__ PipeSlot<C, NW> slot[2];
   __shared__ uint32_t s_draw[2];
+  __shared__ uint32_t s_tag;
   __shared__ __attribute__((aligned(16))) uint16_t stage_all[NW][kStage];
   const int lane = __lane_id();
   const int wid = threadIdx.x / kWave;  // wave NW is the look-back wave
@@ -433,7 +435,7 @@ __global__ __launch_bounds__((NW + 1) * kWave) void scan_kernel(
   // the tile count), so every group has at least one tile and one workgroup.
   const uint32_t K = gridDim.x < (uint32_t)kTicketCounters ? gridDim.x : (uint32_t)kTicketCounters;
   const uint32_t grp = blockIdx.x % K;
-  uint64_t* ctr = ws_ticket(ws, (int)grp);
+  uint64_t* ctr = ws_slot(ws, (int)grp);
   const uint64_t grp_tiles = ((uint64_t)a.num_tiles - grp + K - 1) / K;
   const uint64_t grp_wgs = (gridDim.x - grp + K - 1) / K;
   const uint64_t last_draw = grp_tiles + grp_wgs - 1;
@@ -447,11 +449,13 @@ __global__ __launch_bounds__((NW + 1) * kWave) void scan_kernel(
     slot[threadIdx.x].count = 0;
   }
   if (lb_wave && lane == 0) {
+    s_tag = launch_arrive(ws);
     const uint32_t d0 = draw();
     s_draw[0] = d0;
     s_draw[1] = d0 < a.num_tiles ? draw() : a.num_tiles;
   }
   __syncthreads();
+  a.tag = s_tag;
   uint32_t t = s_draw[0];
   uint32_t tn = s_draw[1];
   if (t >= a.num_tiles) return;
@@ -516,7 +520,7 @@ static uint32_t resident_grid() {
 template <class C, int SEG, int LOC>
 static int launch_scan_tile(int64_t tile, const void* x, void* y, const int8_t* flags, int64_t n,
                             int64_t seg_len, typename C::Acc start, void* ws, uint32_t tiles,
-                            uint32_t tag, hipStream_t st) {
+                            int window, hipStream_t st) {
   const auto* xi = static_cast<const typename C::In*>(x);
   auto* yo = static_cast<typename C::Out*>(y);
   constexpr int G = C::G;
@@ -526,7 +530,7 @@ static int launch_scan_tile(int64_t tile, const void* x, void* y, const int8_t* 
     uint32_t grid = resident_grid<C, NW, J, SEG, LOC>();                                         \
     if (grid > tiles) grid = tiles;                                                              \
     hipLaunchKernelGGL((scan_kernel<C, NW, J, SEG, LOC>), dim3(grid), dim3((NW + 1) * kWave), 0, \
-                       st, xi, yo, flags, start, ws, ScanArgs{n, seg_len, tiles, tag});         \
+                       st, xi, yo, flags, start, ws, ScanArgs{n, seg_len, tiles, 0, window});  \
   } while (0)
   if constexpr (LOC == kLocMfma) {
     switch (groups) {
@@ -558,13 +562,16 @@ static int launch_scan_tile(int64_t tile, const void* x, void* y, const int8_t* 
 // matrix-unit flavour (plain scans of F16 and I8 only) is compiled in its
 // own translation unit and reached through launch_scan_mfma.
 int launch_scan_mfma(int x_dtype, int64_t tile, const void* x, void* y, int64_t n, double start,
-                     void* ws, uint32_t tiles, uint32_t tag, hipStream_t st);
+                     void* ws, uint32_t tiles, int window, hipStream_t st);
 
 template <int SEG>
 static int scan_dispatch(const void* x, int x_dtype, void* y, const int8_t* flags, int64_t n,
                          int64_t seg_len, int S, int tiles_per_wg, double start, void* ws,
-                         size_t ws_bytes, void* stream, int method = TCS_METHOD_AUTO) {
+                         size_t ws_bytes, void* stream, int method = TCS_METHOD_AUTO,
+                         int window = 0) {
   if (n < 0 || S <= 0) return TCS_ERR_INVALID_ARG;
+  if (window == 0) window = kLookbackWindow;
+  if (window < 1 || window > kWave) return TCS_ERR_INVALID_ARG;
   if (method != TCS_METHOD_AUTO && method != TCS_METHOD_DPP && method != TCS_METHOD_MFMA)
     return TCS_ERR_INVALID_ARG;
   if (method == TCS_METHOD_MFMA && (SEG != kSegNone || (x_dtype != TCS_F16 && x_dtype != TCS_I8)))
@@ -596,19 +603,18 @@ static int scan_dispatch(const void* x, int x_dtype, void* y, const int8_t* flag
     return TCS_ERR_UNSUPPORTED_DTYPE;
   hipStream_t st = static_cast<hipStream_t>(stream);
   const uint32_t nt = (uint32_t)tiles;
-  const uint32_t tag = next_launch_tag(ws);
-  if (method == TCS_METHOD_MFMA) return launch_scan_mfma(x_dtype, tile, x, y, n, start, ws, nt, tag, st);
+  if (method == TCS_METHOD_MFMA) return launch_scan_mfma(x_dtype, tile, x, y, n, start, ws, nt, window, st);
   switch (x_dtype) {
     case TCS_F16:
-      return launch_scan_tile<CodecF16, SEG, kLocDpp>(tile, x, y, flags, n, seg_len, (double)(float)start, ws, nt, tag, st);
+      return launch_scan_tile<CodecF16, SEG, kLocDpp>(tile, x, y, flags, n, seg_len, (double)(float)start, ws, nt, window, st);
     case TCS_BF16:
-      return launch_scan_tile<CodecBF16, SEG, kLocDpp>(tile, x, y, flags, n, seg_len, (double)(float)start, ws, nt, tag, st);
+      return launch_scan_tile<CodecBF16, SEG, kLocDpp>(tile, x, y, flags, n, seg_len, (double)(float)start, ws, nt, window, st);
     case TCS_F32:
-      return launch_scan_tile<CodecF32, SEG, kLocDpp>(tile, x, y, flags, n, seg_len, (double)(float)start, ws, nt, tag, st);
+      return launch_scan_tile<CodecF32, SEG, kLocDpp>(tile, x, y, flags, n, seg_len, (double)(float)start, ws, nt, window, st);
     case TCS_I8:
-      return launch_scan_tile<CodecI8, SEG, kLocDpp>(tile, x, y, flags, n, seg_len, (int)start, ws, nt, tag, st);
+      return launch_scan_tile<CodecI8, SEG, kLocDpp>(tile, x, y, flags, n, seg_len, (int)start, ws, nt, window, st);
     default:
-      return launch_scan_tile<CodecI32, SEG, kLocDpp>(tile, x, y, flags, n, seg_len, (int)start, ws, nt, tag, st);
+      return launch_scan_tile<CodecI32, SEG, kLocDpp>(tile, x, y, flags, n, seg_len, (int)start, ws, nt, window, st);
   }
 }
 

@@ -5,13 +5,13 @@
 namespace tcs {
 
 int launch_scan_mfma(int x_dtype, int64_t tile, const void* x, void* y, int64_t n, double start,
-                     void* ws, uint32_t tiles, uint32_t tag, hipStream_t st) {
+                     void* ws, uint32_t tiles, int window, hipStream_t st) {
   if (x_dtype == TCS_F16)
     return launch_scan_tile<CodecF16, kSegNone, kLocMfma>(tile, x, y, nullptr, n, 0, (double)(float)start,
-                                                          ws, tiles, tag, st);
+                                                          ws, tiles, window, st);
   if (x_dtype == TCS_I8)
     return launch_scan_tile<CodecI8, kSegNone, kLocMfma>(tile, x, y, nullptr, n, 0, (int)start, ws,
-                                                         tiles, tag, st);
+                                                         tiles, window, st);
   return TCS_ERR_UNSUPPORTED_DTYPE;
 }
 

@@ -131,15 +131,17 @@ __device__ __forceinline__ int lower_bound_u32(const uint32_t* list, int cnt, ui
 template <class Src, class IdxT, class OutT>
 __global__ __launch_bounds__(kSegNW * kWave) void seg_reduce_kernel(
     Src src, const IdxT* __restrict__ ends, int64_t R, int64_t n_ends, int64_t nnz,
-    OutT* __restrict__ y, const uint32_t* __restrict__ tile_row, uint32_t num_tiles, void* ws,
-    uint32_t tag) {
+    OutT* __restrict__ y, const uint32_t* __restrict__ tile_row, uint32_t num_tiles, void* ws) {
   using Acc = typename Src::Acc;
   constexpr int G = kSegG, J = kSegJ, NW = kSegNW;
   constexpr int64_t GSPAN = (int64_t)kWave * G, WSEG = GSPAN * J;
   __shared__ uint32_t s_list[kEndsCap];
   __shared__ Acc s_wval[NW], s_wexcl[NW], s_carry;
   __shared__ int s_wflag[NW], s_wexclf[NW];
+  __shared__ uint32_t s_tag;
 
+  // Every workgroup arrives, including those that return early below.
+  if (threadIdx.x == 0) s_tag = launch_arrive(ws);
   const int lane = __lane_id();
   const int wid = threadIdx.x / kWave;
   const uint32_t tile = blockIdx.x;
@@ -281,6 +283,7 @@ __global__ __launch_bounds__(kSegNW * kWave) void seg_reduce_kernel(
       return FlagVal<Acc>{s >= u0 ? 1 : 0, wave_sum(sum)};
     };
     uint64_t* desc = ws_desc(ws);
+    const uint32_t tag = s_tag;
     Acc carry = Acc(0);
     if (tile == 0) {
       if (lane == 0 && num_tiles > 1) desc_publish<Acc>(desc, 0, agg, tag, kFlagIncl);
@@ -356,9 +359,8 @@ static int launch_seg(const Src& src, const void* ends_v, int64_t R, int64_t n_e
                      ends, R, n_ends, nnz, y, tile_row, plan.num_tiles);
   TCS_CHECK_LAUNCH();
   if (plan.num_tiles == 0) return TCS_OK;
-  const uint32_t tag = next_launch_tag(ws);
   hipLaunchKernelGGL((seg_reduce_kernel<Src, IdxT, OutT>), dim3(plan.num_tiles), dim3(kSegNW * kWave),
-                     0, st, src, ends, R, n_ends, nnz, y, tile_row, plan.num_tiles, ws, tag);
+                     0, st, src, ends, R, n_ends, nnz, y, tile_row, plan.num_tiles, ws);
   TCS_CHECK_LAUNCH();
   return TCS_OK;
 }

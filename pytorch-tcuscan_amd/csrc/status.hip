@@ -1,23 +1,5 @@
-// Status strings, ABI version and the per-workspace launch counters of
-// libtcuscan_hip.so.
-#include <mutex>
-#include <unordered_map>
-
+// Status strings and ABI version of libtcuscan_hip.so.
 #include "tcs_common.hpp"
-
-namespace tcs {
-
-// Launches on one workspace are stream-ordered by contract, so a counter per
-// workspace address gives every launch a tag distinct from the previous
-// 2^30 - 1 launches on it.
-uint32_t next_launch_tag(const void* ws) {
-  static std::mutex mu;
-  static std::unordered_map<const void*, uint32_t> epochs;
-  std::lock_guard<std::mutex> lock(mu);
-  return tag_of_epoch(epochs[ws]++);
-}
-
-}  // namespace tcs
 
 extern "C" {
 
@@ -32,6 +14,6 @@ const char* tcs_status_string(int status) {
   }
 }
 
-int tcs_abi_version(void) { return 100; }
+int tcs_abi_version(void) { return 101; }
 
 }  // extern "C"

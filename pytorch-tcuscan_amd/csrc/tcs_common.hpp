@@ -129,16 +129,21 @@ __device__ __forceinline__ void reg_barrier(T (&r)[ITEMS]) {
 }
 
 // ---------------------------------------------------------------------------
-// Workspace layout.  Every workspace starts with a 2 KiB header holding the
-// persistent scan's kTicketCounters ticket counters, 256 bytes apart so that
-// their atomics do not serialise on one address; they are zero between
-// calls (each scan launch resets them) and no op puts scratch data there.
-// Look-back kernels keep 16 bytes of descriptor per tile right after the
-// header; other scratch follows the descriptors.
+// Workspace layout.  Every workspace starts with a header of 18 slots of
+// 256 bytes (one word each, apart so that their atomics do not serialise on
+// one address):
+//   slots 0-7    ticket counters of the persistent scan;
+//   slot 8       the launch epoch (see launch_arrive);
+//   slot 9       groups arrived, slots 10-17: workgroups arrived per group.
+// All but the epoch are zero between calls (each launch resets them), and
+// no op puts scratch data in the header.  Look-back kernels keep 16 bytes
+// of descriptor per tile right after the header; other scratch follows the
+// descriptors.
 // ---------------------------------------------------------------------------
 constexpr int kTicketCounters = 8;
-constexpr size_t kTicketStride = 256;
-constexpr size_t kHeaderBytes = kTicketCounters * kTicketStride;
+constexpr int kSlotEpoch = 8, kSlotGroupsArrived = 9, kSlotArrived0 = 10;
+constexpr size_t kSlotBytes = 256;
+constexpr size_t kHeaderBytes = (kSlotArrived0 + kTicketCounters) * kSlotBytes;
 constexpr size_t kDescBytes = 16;
 
 inline size_t lookback_ws_bytes(uint64_t num_tiles) {
@@ -146,11 +151,36 @@ inline size_t lookback_ws_bytes(uint64_t num_tiles) {
   return (b + 255) & ~(size_t)255;
 }
 
-__device__ __forceinline__ uint64_t* ws_ticket(void* ws, int k) {
-  return reinterpret_cast<uint64_t*>(reinterpret_cast<char*>(ws) + (size_t)k * kTicketStride);
+__device__ __forceinline__ uint64_t* ws_slot(void* ws, int k) {
+  return reinterpret_cast<uint64_t*>(reinterpret_cast<char*>(ws) + (size_t)k * kSlotBytes);
 }
 __device__ __forceinline__ uint64_t* ws_desc(void* ws) {
   return reinterpret_cast<uint64_t*>(reinterpret_cast<char*>(ws) + kHeaderBytes);
+}
+
+// Launch tag of the current launch on workspace `ws`, read by one thread of
+// every workgroup at its start.  The tag comes from the epoch word in the
+// workspace, so it is right under HIP graph replay too (a tag passed as a
+// kernel argument would be frozen into the graph).  Each workgroup then
+// arrives on its group's counter (group = blockIdx.x % K); the last arrival
+// of a group arrives on the groups counter, and the last group advances
+// the epoch for the next launch.  Every workgroup has read the epoch before
+// it arrives (release), so the advance happens after all reads.
+__device__ __forceinline__ uint32_t launch_arrive(void* ws) {
+  const uint64_t e = __hip_atomic_load(ws_slot(ws, kSlotEpoch), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const uint32_t K = gridDim.x < (uint32_t)kTicketCounters ? gridDim.x : (uint32_t)kTicketCounters;
+  const uint32_t g = blockIdx.x % K;
+  const uint64_t members = (gridDim.x - g + K - 1) / K;
+  uint64_t* c = ws_slot(ws, kSlotArrived0 + (int)g);
+  if (__hip_atomic_fetch_add(c, 1ull, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) == members - 1) {
+    __hip_atomic_store(c, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint64_t* d = ws_slot(ws, kSlotGroupsArrived);
+    if (__hip_atomic_fetch_add(d, 1ull, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) == K - 1) {
+      __hip_atomic_store(d, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(ws_slot(ws, kSlotEpoch), e + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+  return tag_of_epoch((uint32_t)(e % 0x3FFFFFFFull));
 }
 
 // Tile length (elements) for a look-back kernel: tiles_per_wg matrix tiles
@@ -172,9 +202,10 @@ inline int64_t tile_elems_for(int S, int tiles_per_wg) {
 
 inline bool aligned16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
 
-// Launch tag for the next look-back launch on workspace `ws` (one counter
-// per workspace address, kept on the host; see tag_of_epoch).
-uint32_t next_launch_tag(const void* ws);
+// The flag-segmented scan with an explicit look-back window (scan_flags.hip).
+int seg_scan_flags_cfg(const void* x, int x_dtype, const int8_t* flags, void* y, int64_t n, int S,
+                       int tiles_per_wg, int window, void* workspace, size_t workspace_bytes,
+                       void* stream);
 
 #define TCS_CHECK_LAUNCH()                                  \
   do {                                                      \

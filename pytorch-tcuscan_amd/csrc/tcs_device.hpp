@@ -136,12 +136,12 @@ __device__ __forceinline__ T wave_sum(T v) {
 // ---------------------------------------------------------------------------
 // Launch tags.
 //
-// Every launch on a workspace gets a tag
-// in [1, 2^30 - 1] that differs from the tags of recent launches on that
-// workspace (the host keeps one counter per workspace), and descriptors carry
-// it, so descriptors left by earlier launches are never mistaken for current
-// ones and the workspace needs no clearing between calls.  Zeroed memory
-// carries tag 0, which no launch uses.
+// Every launch on a workspace gets a tag in [1, 2^30 - 1] that differs from
+// the tags of recent launches on that workspace (an epoch kept in the
+// workspace advances once per launch), and descriptors carry it, so
+// descriptors left by earlier launches are never mistaken for current ones
+// and the workspace needs no clearing between calls.  Zeroed memory carries
+// tag 0, which no launch uses.
 // ---------------------------------------------------------------------------
 __host__ __device__ inline uint32_t tag_of_epoch(uint32_t epoch) { return epoch % 0x3FFFFFFFu + 1u; }
 
@@ -198,9 +198,9 @@ __device__ __forceinline__ bool desc_try_read(const uint64_t* desc, int64_t tile
 
 // Exclusive prefix of `tile`: the sum of predecessor descriptors back to and
 // including the nearest inclusive one.  Called by one whole wavefront; lane
-// l < WIN inspects tile - 1 - l, so every poll covers WIN predecessors (a
-// 16-wide window measured faster than 64 on gfx950: fewer device-scope
-// loads on the hot descriptors near the frontier).  Tiles before index 0
+// l < win (1 <= win <= 64) inspects tile - 1 - l, so every poll covers win
+// predecessors (a 16-wide window measured faster than 64 on gfx950: fewer
+// device-scope loads on the hot descriptors near the frontier).  Tiles before index 0
 // read as an inclusive zero, which bounds the walk.
 //
 // Forward progress does not rely on the order in which the hardware starts
@@ -218,8 +218,9 @@ struct FlagVal {
   T value;
 };
 
-template <class T, int WIN = kLookbackWindow, class Fallback>
-__device__ T lookback(const uint64_t* desc, uint32_t tile, uint32_t tag, Fallback&& fallback) {
+template <class T, class Fallback>
+__device__ T lookback(const uint64_t* desc, uint32_t tile, uint32_t tag, Fallback&& fallback,
+                      int win = kLookbackWindow) {
   const int lane = __lane_id();
   T acc = T(0);
   int64_t base = (int64_t)tile - 1;
@@ -228,7 +229,7 @@ __device__ T lookback(const uint64_t* desc, uint32_t tile, uint32_t tag, Fallbac
     T val = T(0);
     int flag = (int)kFlagIncl;
     bool ok = true;
-    if (lane >= WIN) {
+    if (lane >= win) {
       flag = (int)kFlagAgg;  // outside the window: contributes nothing
     } else if (idx >= 0) {
       ok = desc_try_read(desc, idx, tag, val, flag);
@@ -262,7 +263,7 @@ __device__ T lookback(const uint64_t* desc, uint32_t tile, uint32_t tag, Fallbac
       __builtin_amdgcn_s_sleep(1);
       if (!ok) ok = desc_try_read(desc, idx, tag, val, flag);
     }
-    base -= WIN;
+    base -= win;
   }
 }
 

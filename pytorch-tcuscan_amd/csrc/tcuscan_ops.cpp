@@ -18,7 +18,9 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <string>
 #include <tuple>
+#include <vector>
 #include <utility>
 
 #include "../../include/tcuscan.h"
@@ -149,11 +151,12 @@ at::Tensor run_scan_multi_core_no_l2(const at::Tensor& x, int S) {
 // (tiles_per_wg S x S tiles, 0 = default) and local scan method ("auto",
 // "dpp" or "mfma"); every choice gives bit-identical results.
 at::Tensor run_scan_ex(const at::Tensor& x_in, int S, int64_t tiles_per_wg, const std::string& method,
-                       double starting_sum) {
+                       double starting_sum, int64_t lookback_window) {
   const char* op = "run_scan_ex";
   require_device(x_in, op, "x");
   TORCH_CHECK(S > 0, op, ": S must be positive");
   TORCH_CHECK(tiles_per_wg >= 0, op, ": tiles_per_wg must be non-negative");
+  TORCH_CHECK(lookback_window >= 0 && lookback_window <= 64, op, ": lookback_window must be in [0, 64]");
   int m = TCS_METHOD_AUTO;
   if (method == "dpp") {
     m = TCS_METHOD_DPP;
@@ -171,8 +174,8 @@ at::Tensor run_scan_ex(const at::Tensor& x_in, int S, int64_t tiles_per_wg, cons
   const int tpw = (int)tiles_per_wg;
   auto ws = workspace(x, tcs_scan_workspace_bytes(n, S, tpw));
   const at::hip::HIPGuardMasqueradingAsCUDA guard(x.device());
-  check(tcs_scan_method(x.data_ptr(), dtype_code(t), z.data_ptr(), n, S, tpw, starting_sum, m, ws.first,
-                        ws.second, stream_of(x)),
+  check(tcs_scan_method(x.data_ptr(), dtype_code(t), z.data_ptr(), n, S, tpw, starting_sum, m,
+                        (int)lookback_window, ws.first, ws.second, stream_of(x)),
         op);
   return z;
 }
@@ -677,9 +680,163 @@ at::Tensor run_seg_sum(const at::Tensor& x_in, const at::Tensor& f_in, int S) {
 
 int abi_version() { return tcs_abi_version(); }
 
+// ---------------------------------------------------------------------------
+// Batched work plans (no counterpart in the reference): the C ABI's
+// tcs_batch with torch tensors.  Inputs are bound by address, so they must
+// be contiguous, 16-byte aligned GPU tensors that the caller keeps (the plan
+// holds references); outputs are allocated by the add_* calls.
+// ---------------------------------------------------------------------------
+class BatchPlan {
+ public:
+  BatchPlan() { check(tcs_batch_create(&b_), "BatchPlan"); }
+  ~BatchPlan() { tcs_batch_destroy(b_); }
+  BatchPlan(const BatchPlan&) = delete;
+  BatchPlan& operator=(const BatchPlan&) = delete;
+
+  at::Tensor add_scan(const at::Tensor& x) {
+    const char* op = "BatchPlan.add_scan";
+    bind(x, op, "x");
+    const auto t = x.scalar_type();
+    at::Tensor y = at::empty({x.numel()}, x.options().dtype(scan_out_dtype(t)));
+    added(tcs_batch_add_scan(b_, x.data_ptr(), dtype_code(t), y.data_ptr(), x.numel()), op, {x, y});
+    return y;
+  }
+
+  at::Tensor add_seg_scan(const at::Tensor& x, const at::Tensor& flags) {
+    const char* op = "BatchPlan.add_seg_scan";
+    bind(x, op, "x");
+    bind(flags, op, "flags");
+    TORCH_CHECK(flags.scalar_type() == at::kChar && flags.numel() == x.numel(), op,
+                ": flags must be int8 with one flag per element");
+    const auto t = x.scalar_type();
+    at::Tensor y = at::empty({x.numel()}, x.options().dtype(scan_out_dtype(t)));
+    added(tcs_batch_add_seg_scan(b_, x.data_ptr(), dtype_code(t),
+                                 static_cast<const int8_t*>(flags.data_ptr()), y.data_ptr(), x.numel()),
+          op, {x, flags, y});
+    return y;
+  }
+
+  // Full CSR indptr (num_segments + 1 entries), as run_seg_sum_multi_core.
+  at::Tensor add_seg_sum(const at::Tensor& x, const at::Tensor& indptr) {
+    const char* op = "BatchPlan.add_seg_sum";
+    bind(x, op, "x");
+    bind(indptr, op, "indptr");
+    TORCH_CHECK(index_code(indptr) >= 0 && indptr.numel() >= 2, op,
+                ": indptr must be int32, uint32 or int64 with at least two entries");
+    const int64_t R = indptr.numel() - 1;
+    at::Tensor y = at::empty({R}, x.options().dtype(scan_out_dtype(x.scalar_type())));
+    const void* ends = static_cast<const char*>(indptr.data_ptr()) + indptr.element_size();
+    added(tcs_batch_add_seg_sum(b_, x.data_ptr(), dtype_code(x.scalar_type()), ends, index_code(indptr), R,
+                                R, y.data_ptr(), x.numel()),
+          op, {x, indptr, y});
+    return y;
+  }
+
+  at::Tensor add_spmv(const at::Tensor& vals, const at::Tensor& indptr, const at::Tensor& cols,
+                      const at::Tensor& x) {
+    const char* op = "BatchPlan.add_spmv";
+    bind(vals, op, "vals");
+    bind(indptr, op, "indptr");
+    bind(cols, op, "cols");
+    bind(x, op, "x");
+    TORCH_CHECK(cols.scalar_type() == at::kInt && cols.numel() == vals.numel(), op,
+                ": cols must be int32, one per value");
+    TORCH_CHECK(index_code(indptr) >= 0 && indptr.numel() >= 2, op, ": bad indptr");
+    TORCH_CHECK(x.scalar_type() == vals.scalar_type(), op, ": x and vals must share a dtype");
+    const int64_t R = indptr.numel() - 1;
+    at::Tensor y = at::empty({R}, vals.options().dtype(at::kFloat));
+    const void* ends = static_cast<const char*>(indptr.data_ptr()) + indptr.element_size();
+    added(tcs_batch_add_spmv(b_, vals.data_ptr(), dtype_code(vals.scalar_type()), ends, index_code(indptr),
+                             static_cast<const int32_t*>(cols.data_ptr()), x.data_ptr(), x.numel(),
+                             static_cast<float*>(y.data_ptr()), R, vals.numel()),
+          op, {vals, indptr, cols, x, y});
+    return y;
+  }
+
+  int num_ops() const { return tcs_batch_num_ops(b_); }
+
+  void tune(int reps, bool force) {
+    ensure_ws();
+    const at::hip::HIPGuardMasqueradingAsCUDA guard(ws_.device());
+    check(tcs_batch_tune(b_, ws_.data_ptr(), (size_t)ws_.numel(), reps, force ? 1 : 0, stream_of(ws_)),
+          "BatchPlan.tune");
+  }
+
+  void run_eager() {
+    ensure_ws();
+    const at::hip::HIPGuardMasqueradingAsCUDA guard(ws_.device());
+    check(tcs_batch_run_eager(b_, ws_.data_ptr(), (size_t)ws_.numel(), stream_of(ws_)), "BatchPlan.run_eager");
+  }
+
+  void instantiate(int max_branches) {
+    ensure_ws();
+    const at::hip::HIPGuardMasqueradingAsCUDA guard(ws_.device());
+    check(tcs_batch_instantiate(b_, max_branches, ws_.data_ptr(), (size_t)ws_.numel(), stream_of(ws_)),
+          "BatchPlan.instantiate");
+  }
+
+  void launch() {
+    TORCH_CHECK(ws_.defined(), "BatchPlan.launch: call instantiate() first");
+    const at::hip::HIPGuardMasqueradingAsCUDA guard(ws_.device());
+    check(tcs_batch_launch(b_, stream_of(ws_)), "BatchPlan.launch");
+  }
+
+  // Per op: S, tiles_per_wg, method, window and the tuner's best time.
+  std::vector<std::tuple<int, int, std::string, int, double>> configs() const {
+    std::vector<std::tuple<int, int, std::string, int, double>> out;
+    for (int i = 0; i < tcs_batch_num_ops(b_); ++i) {
+      int S = 0, tpw = 0, m = 0, w = 0;
+      float ms = -1.0f;
+      check(tcs_batch_op_config(b_, i, &S, &tpw, &m, &w, &ms), "BatchPlan.configs");
+      const char* mn = m == TCS_METHOD_DPP ? "dpp" : m == TCS_METHOD_MFMA ? "mfma" : "auto";
+      out.emplace_back(S, tpw, std::string(mn), w, (double)ms);
+    }
+    return out;
+  }
+
+ private:
+  void bind(const at::Tensor& t, const char* op, const char* name) {
+    require_device(t, op, name);
+    TORCH_CHECK(t.is_contiguous() && (reinterpret_cast<uintptr_t>(t.data_ptr()) & 15u) == 0, op, ": ",
+                name, " must be contiguous and 16-byte aligned (the plan binds its address)");
+    TORCH_CHECK(t.numel() > 0, op, ": ", name, " must not be empty");
+    if (device_ < 0) device_ = t.device().index();
+    TORCH_CHECK(t.device().index() == device_, op, ": all tensors of a plan must be on one device");
+  }
+  void added(int rc, const char* op, std::initializer_list<at::Tensor> keep) {
+    TORCH_CHECK(rc >= 0, op, ": ", tcs_status_string(-rc));
+    for (const auto& t : keep) keep_.push_back(t);
+  }
+  void ensure_ws() {
+    TORCH_CHECK(device_ >= 0, "BatchPlan: no ops added");
+    const size_t need = tcs_batch_workspace_bytes(b_);
+    if (!ws_.defined() || (size_t)ws_.numel() < need)
+      ws_ = at::zeros({(int64_t)need}, at::TensorOptions().dtype(at::kByte).device(at::kCUDA, device_));
+  }
+
+  tcs_batch* b_ = nullptr;
+  std::vector<at::Tensor> keep_;
+  at::Tensor ws_;
+  int device_ = -1;
+};
+
 }  // namespace tcuscan
 
 PYBIND11_MODULE(tcuscan_ops, m) {
+  py::class_<tcuscan::BatchPlan>(m, "BatchPlan",
+                                 "Batched scans / seg-scans / seg-sums / SpMVs replayed as one HIP graph")
+      .def(py::init<>())
+      .def("add_scan", &tcuscan::BatchPlan::add_scan, py::arg("x"))
+      .def("add_seg_scan", &tcuscan::BatchPlan::add_seg_scan, py::arg("x"), py::arg("flags"))
+      .def("add_seg_sum", &tcuscan::BatchPlan::add_seg_sum, py::arg("x"), py::arg("indptr"))
+      .def("add_spmv", &tcuscan::BatchPlan::add_spmv, py::arg("vals"), py::arg("indptr"), py::arg("cols"),
+           py::arg("x"))
+      .def("num_ops", &tcuscan::BatchPlan::num_ops)
+      .def("tune", &tcuscan::BatchPlan::tune, py::arg("reps") = 3, py::arg("force") = false)
+      .def("run_eager", &tcuscan::BatchPlan::run_eager)
+      .def("instantiate", &tcuscan::BatchPlan::instantiate, py::arg("max_branches") = 4)
+      .def("launch", &tcuscan::BatchPlan::launch)
+      .def("configs", &tcuscan::BatchPlan::configs);
   namespace py = pybind11;
   m.doc() = "tcuscan operators for AMD Instinct MI355X (HIP, gfx950)";
   m.def("abi_version", &tcuscan::abi_version, "C ABI version of libtcuscan_hip.so");
@@ -693,8 +850,8 @@ PYBIND11_MODULE(tcuscan_ops, m) {
   m.def("run_block_scan", &tcuscan::run_block_scan, "Block scan on blocks of length S^2");
   m.def("run_scan_multi_cube", &tcuscan::run_scan_multi_cube, "Multi-cube scan");
   m.def("run_scan_ex", &tcuscan::run_scan_ex, py::arg("x"), py::arg("S"), py::arg("tiles_per_wg") = 0,
-        py::arg("method") = "auto", py::arg("starting_sum") = 0.0,
-        "Scan with an explicit tile size and local scan method ('auto', 'dpp', 'mfma')");
+        py::arg("method") = "auto", py::arg("starting_sum") = 0.0, py::arg("lookback_window") = 0,
+        "Scan with an explicit tile size, local scan method ('auto', 'dpp', 'mfma') and look-back window");
   m.def("run_seg_scan", &tcuscan::run_seg_scan, "Segmented Scan");
   m.def("run_seg_scan_vec", &tcuscan::run_seg_scan_vec, "Segmented Scan (vector-only)");
   m.def("run_scan_cpu", &tcuscan::run_scan_cpu, "Scan on CPUs");

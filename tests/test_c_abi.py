@@ -66,16 +66,18 @@ def test_scan_argument_validation(lib):
 def test_scan_method_argument_validation(lib):
     f = lib.tcs_scan_method
     f.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int,
-                  ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_void_p, ctypes.c_size_t,
-                  ctypes.c_void_p]
+                  ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                  ctypes.c_size_t, ctypes.c_void_p]
     fake = 0x100000
     args = (fake, 0, fake, 1 << 20, 128, 0, 0.0)
-    assert f(*args, 3, fake, 1 << 24, None) == TCS_ERR_INVALID_ARG          # no such method
-    assert f(*args, -1, fake, 1 << 24, None) == TCS_ERR_INVALID_ARG
+    assert f(*args, 3, 0, fake, 1 << 24, None) == TCS_ERR_INVALID_ARG          # no such method
+    assert f(*args, -1, 0, fake, 1 << 24, None) == TCS_ERR_INVALID_ARG
+    assert f(*args, 1, 65, fake, 1 << 24, None) == TCS_ERR_INVALID_ARG         # window > 64
+    assert f(*args, 1, -2, fake, 1 << 24, None) == TCS_ERR_INVALID_ARG
     for dt in (1, 2, 5):  # bf16, f32, i32: no matrix-unit path
-        assert f(fake, dt, fake, 1 << 20, 128, 0, 0.0, 2, fake, 1 << 24, None) == TCS_ERR_UNSUPPORTED_DTYPE
-    assert f(*args, 2, fake, 256, None) == TCS_ERR_WORKSPACE
-    assert f(fake, 0, fake, 0, 128, 0, 0.0, 2, None, 0, None) == TCS_OK  # empty: no launch
+        assert f(fake, dt, fake, 1 << 20, 128, 0, 0.0, 2, 0, fake, 1 << 24, None) == TCS_ERR_UNSUPPORTED_DTYPE
+    assert f(*args, 2, 0, fake, 256, None) == TCS_ERR_WORKSPACE
+    assert f(fake, 0, fake, 0, 128, 0, 0.0, 2, 0, None, 0, None) == TCS_OK  # empty: no launch
 
 
 def test_scan_rows_argument_validation(lib):
