@@ -239,6 +239,80 @@ def _batched_workload(steps: int, warmup: int, num_scans: int = 10_000, num_segs
     }
 
 
+def _serve_workload(seconds: float, rate: float, seed: int = 0):
+    """The serving stream: open-loop Poisson arrivals at `rate` req/s for
+    `seconds`; mix 40% scans (fp16 and int8, N = 2^U(10, 26)), 30% CSR SpMV
+    (fp16, lognormal rows of mean 16 clipped to 20000, nnz 10^U(4, 7), at least
+    2^16 columns), 20% seg-sums (fp16,
+    geometric segments of mean 64, nnz 10^U(4, 7)), 10% compaction (fp32,
+    N = 2^U(10, 24), Bernoulli mask p ~ U(0.05, 0.9)).  Inputs are a
+    device-resident pool of distinct requests per kind (requests draw pool
+    entries uniformly); admission under a 64 GB HBM budget."""
+    import numpy as np
+    import torch
+
+    import tcuscan
+    from tcuscan import workloads
+
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(seed)
+    srv = tcuscan.Server(64 << 30, 4, 32)
+    kinds = {"scan": [], "spmv": [], "segsum": [], "compress": []}
+    hold = []
+    for i in range(48):
+        n = int(2 ** rng.uniform(10, 26))
+        g = torch.Generator(device=dev).manual_seed(1000 + i)
+        if i % 2 == 0:
+            x = (torch.rand(n, generator=g, device=dev) * 2 - 1).half()
+        else:
+            x = torch.randint(-128, 128, (n,), generator=g, device=dev, dtype=torch.int8)
+        hold.append(x)
+        kinds["scan"].append(srv.add_scan(x))
+    for i in range(16):
+        nnz = 10 ** rng.uniform(4, 7)
+        rows = max(16, int(nnz / 16))
+        cols = max(rows, 1 << 16)  # room for distinct columns in the longest rows
+        m = workloads.lognormal_csr(rows, cols, 16.0, 1.2, 20000, seed=2000 + i, device=dev)
+        hold.append(m)
+        kinds["spmv"].append(srv.add_spmv(*m))
+    for i in range(16):
+        nnz = 10 ** rng.uniform(4, 7)
+        lg = float(np.log2(max(2.0, nnz / 64)))
+        ss = workloads.geometric_segsums(1, 64.0, lg, lg + 1e-6, seed=3000 + i, device=dev)
+        x = ss.x.half()
+        hold.append((x, ss.indptr))
+        kinds["segsum"].append(srv.add_seg_sum(x, ss.indptr))
+    for i in range(16):
+        n = int(2 ** rng.uniform(10, 24))
+        p = rng.uniform(0.05, 0.9)
+        g = torch.Generator(device=dev).manual_seed(4000 + i)
+        x = torch.rand(n, generator=g, device=dev)
+        mk = (torch.rand(n, generator=g, device=dev) < p).to(torch.int8)
+        hold.append((x, mk))
+        kinds["compress"].append(srv.add_compress(x, mk))
+    torch.cuda.synchronize()
+    n_req = int(rate * seconds)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n_req))
+    pick = rng.choice(4, n_req, p=[0.4, 0.3, 0.2, 0.1])
+    names = ["scan", "spmv", "segsum", "compress"]
+    entries = [int(rng.choice(kinds[names[k]])) for k in pick]
+    srv.run(arrivals[: min(200, n_req)].tolist(), entries[: min(200, n_req)])  # warm-up
+    st = srv.run(arrivals.tolist(), entries)
+    lat = np.asarray(st.pop("latency_s"))
+    per_kind = {}
+    for k, nm in enumerate(names):
+        sel = lat[pick == k]
+        if sel.size:
+            per_kind[nm] = {"n": int(sel.size), "p50_ms": round(float(np.percentile(sel, 50)) * 1e3, 4),
+                            "p99_ms": round(float(np.percentile(sel, 99)) * 1e3, 4)}
+    st["per_kind"] = per_kind
+    st["offered_rps"] = rate
+    st["seconds"] = seconds
+    del srv, hold
+    torch.cuda.empty_cache()
+    return st
+
+
 def _cpu_baseline(n: int, budget_s: float = 10.0):
     """Times the oracle's C port of run_scan_cpu on the full fp16 workload."""
     import numpy as np
@@ -281,9 +355,12 @@ def main() -> int:
     ap.add_argument("--S", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-extras", action="store_true", help="skip the extra workloads")
+    ap.add_argument("--serve-seconds", type=float, default=10.0,
+                    help="length of the serving stream (the full workload is 60 s)")
+    ap.add_argument("--serve-rate", type=float, default=2000.0, help="Poisson arrival rate, req/s")
     ap.add_argument("--only", choices=["scan_f16", "scan_i8", "scan_bf16", "scan_f16_mfma", "scan_i8_mfma",
                                        "scan_f16_dpp", "scan_i8_dpp", "compress_ind", "spmv", "copy_f32",
-                                       "batched"],
+                                       "batched", "serve"],
                     help="run one workload and print its timing only (for profilers)")
     a = ap.parse_args()
     if a.gpus != 1:
@@ -304,6 +381,8 @@ def main() -> int:
             r = _spmv_workload(1 << 22, a.steps, a.warmup)
         elif a.only == "batched":
             r = _batched_workload(a.steps, a.warmup)
+        elif a.only == "serve":
+            r = _serve_workload(a.serve_seconds, a.serve_rate)
         else:
             r = _copy_workload(n, a.steps, a.warmup)
         print(json.dumps({"only": a.only, **{k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()}}))
@@ -340,6 +419,14 @@ def main() -> int:
             "batched_ms": round(bt["batched_ms"], 3), "per_call_ms": round(bt["per_call_ms"], 2),
             "gbs_batched": round(bt["gbs_batched"], 1), "scan_elems": bt["scan_elems"],
             "segsum_nnz": bt["segsum_nnz"], "tuned": bt["tuned"], "same_results": bt["same_results"],
+        }
+        sv = _serve_workload(a.serve_seconds, a.serve_rate)
+        extras["serve_mixed_poisson"] = {
+            "value": round(sv["p99_ms"], 3), "unit": "ms p99 latency",
+            "p50_ms": round(sv["p50_ms"], 3), "p99_ms": round(sv["p99_ms"], 3),
+            "throughput_rps": round(sv["throughput_rps"], 1), "offered_rps": sv["offered_rps"],
+            "gbs": round(sv["gbs"], 2), "requests": sv["requests"], "seconds": sv["seconds"],
+            "batches": sv["batches"], "per_kind": sv["per_kind"],
         }
     cpu = None if a.no_cpu else _cpu_baseline(n)
 

@@ -820,9 +820,12 @@ class BatchPlan {
   int device_ = -1;
 };
 
+void register_server(py::module& m);  // serve.cpp
+
 }  // namespace tcuscan
 
 PYBIND11_MODULE(tcuscan_ops, m) {
+  tcuscan::register_server(m);
   py::class_<tcuscan::BatchPlan>(m, "BatchPlan",
                                  "Batched scans / seg-scans / seg-sums / SpMVs replayed as one HIP graph")
       .def(py::init<>())

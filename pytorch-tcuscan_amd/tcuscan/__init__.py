@@ -26,4 +26,4 @@ except ImportError as exc:  # pragma: no cover - exercised only without a build
 
 from tcuscan_ops import *  # noqa: E402,F401,F403
 
-__all__ = [n for n in dir(tcuscan_ops) if n.startswith("run_")] + ["BatchPlan"]
+__all__ = [n for n in dir(tcuscan_ops) if n.startswith("run_")] + ["BatchPlan", "Server"]
