@@ -92,7 +92,8 @@ enum tcs_scan_method {
 };
 
 /* tcs_scan with an explicit local scan method and look-back window
- * (predecessor tiles polled per step, 1..64; 0 = default 16); results are
+ * (predecessor tiles polled per step, 1..64; 0 = the default: 8 for DPP, 32
+ * for MFMA); results are
  * identical for every choice (bit-exact).  TCS_METHOD_MFMA multiplies each 1024-element
  * unit, as 32 rows of 32, by a lower-triangular ones matrix on
  * v_mfma_f32_32x32x16_f16 (fp16, split into two exactly-summed parts) or

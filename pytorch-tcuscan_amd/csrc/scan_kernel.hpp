@@ -171,14 +171,13 @@ __device__ __forceinline__ void tile_load(TileRegs<C, J>& r, const typename C::I
     r.heads[j] = group_heads<C, SEG, Gm::G>(flags, wbase + j * Gm::GSPAN, a.n, a.seg_len, FULL);
 }
 
-// Data-wave reduction of a tile: the wave's (segmented) total into the
-// slot.  Plain scans sum every lane's elements and reduce once across the
-// wave; segmented scans scan each group across the wave (segment order
-// matters).  The last data wave to arrive scans the wave totals and
-// publishes the tile's descriptor: inclusive for tile 0 and for tiles
-// holding a segment head, aggregate otherwise.
+// Data-wave reduction of a tile: lane-serial sums, one wave scan per group,
+// the wave total into the slot.  The last data wave to arrive scans the
+// wave totals and publishes the tile's descriptor: inclusive for tile 0 and
+// for tiles holding a segment head, aggregate otherwise.
 template <class C, int NW, int J, int SEG, int LOC>
-__device__ __forceinline__ void tile_reduce(const TileRegs<C, J>& r, PipeSlot<C, NW>& sl,
+__device__ __forceinline__ void tile_reduce(const TileRegs<C, J>& r, typename C::Acc (&gex)[J],
+                                            int (&gexf)[J], PipeSlot<C, NW>& sl,
                                             typename C::Acc start, void* ws, const ScanArgs& a,
                                             uint32_t tile, int wid, int lane) {
   using Acc = typename C::Acc;
@@ -187,7 +186,9 @@ __device__ __forceinline__ void tile_reduce(const TileRegs<C, J>& r, PipeSlot<C,
   static_assert(LOC == kLocDpp || !kSeg, "the matrix-unit scan covers plain scans only");
   Acc wtot = Acc(0);
   int wtotf = 0;
-  if constexpr (!kSeg) {
+  if constexpr (LOC == kLocMfma) {
+    // Only the wave total is needed here; the units are scanned in the
+    // output pass.
     Acc s = Acc(0);
 #pragma unroll
     for (int j = 0; j < J; ++j)
@@ -196,7 +197,7 @@ __device__ __forceinline__ void tile_reduce(const TileRegs<C, J>& r, PipeSlot<C,
     wtot = wave_sum(s);
   }
 #pragma unroll
-  for (int j = 0; j < (kSeg ? J : 0); ++j) {
+  for (int j = 0; j < (LOC == kLocMfma ? 0 : J); ++j) {
     Acc tv = Acc(0);
     int tf = 0;
 #pragma unroll
@@ -211,9 +212,17 @@ __device__ __forceinline__ void tile_reduce(const TileRegs<C, J>& r, PipeSlot<C,
     }
     int wf = tf;
     Acc wv = tv;
-    wave_seg_incl_scan(wf, wv);
+    if constexpr (kSeg) {
+      wave_seg_incl_scan(wf, wv);
+    } else {
+      wv = wave_incl_scan(wv);
+    }
+    const Acc ev = dpp_mov<kWaveShr1>(wv);
+    const int ef = kSeg ? dpp_i32<kWaveShr1>(wf) : 0;
+    gex[j] = ef ? ev : wtot + ev;
+    gexf[j] = wtotf | ef;
     const Acc gt = readlane(wv, kWave - 1);
-    const int gf = readlane(wf, kWave - 1);
+    const int gf = kSeg ? readlane(wf, kWave - 1) : 0;
     wtot = gf ? gt : wtot + gt;
     wtotf |= gf;
   }
@@ -284,11 +293,10 @@ __device__ __forceinline__ void tile_lookback(PipeSlot<C, NW>& sl,
   if (lane == 0) sl.carry = carry;
 }
 
-// Second pass over the tile: each group's lane prefixes are recomputed from
-// the raw inputs (cheaper than keeping them live across the look-back),
-// the carries added, and the outputs stored.
+// Second pass over the tile: add the carries and store.
 template <class C, int NW, int J, int SEG, bool FULL>
-__device__ __forceinline__ void tile_output(TileRegs<C, J>& r, const PipeSlot<C, NW>& sl,
+__device__ __forceinline__ void tile_output(TileRegs<C, J>& r, const typename C::Acc (&gex)[J],
+                                            const int (&gexf)[J], const PipeSlot<C, NW>& sl,
                                             typename C::Out* __restrict__ y, int64_t n,
                                             uint32_t tile, int wid, int lane) {
   using Acc = typename C::Acc;
@@ -306,38 +314,9 @@ __device__ __forceinline__ void tile_output(TileRegs<C, J>& r, const PipeSlot<C,
     cin = sl.wexclf[wid] ? wx : cin + wx;
   }
   const int64_t wbase = (int64_t)tile * Gm::TILE + (int64_t)wid * Gm::WSEG + (int64_t)lane * G;
-  Acc wrun = Acc(0);  // the wave's (segmented) sum before group j
-  int wrunf = 0;
 #pragma unroll
   for (int j = 0; j < J; ++j) {
-    Acc tv = Acc(0);
-    int tf = 0;
-#pragma unroll
-    for (int i = 0; i < G; ++i) {
-      const Acc v = C::conv(r.raw[j][i]);
-      if (kSeg && (r.heads[j] >> i & 1u)) {
-        tv = v;
-        tf = 1;
-      } else {
-        tv = tv + v;
-      }
-    }
-    int wf = tf;
-    Acc wv = tv;
-    if constexpr (kSeg) {
-      wave_seg_incl_scan(wf, wv);
-    } else {
-      wv = wave_incl_scan(wv);
-    }
-    const Acc ev = dpp_mov<kWaveShr1>(wv);
-    const int ef = kSeg ? dpp_i32<kWaveShr1>(wf) : 0;
-    const Acc gex = ef ? ev : wrun + ev;
-    const int gexf = wrunf | ef;
-    const Acc gt = readlane(wv, kWave - 1);
-    const int gf = kSeg ? readlane(wf, kWave - 1) : 0;
-    wrun = gf ? gt : wrun + gt;
-    wrunf |= gf;
-    Acc run = gexf ? gex : cin + gex;
+    Acc run = gexf[j] ? gex[j] : cin + gex[j];
     Out o[G];
 #pragma unroll
     for (int i = 0; i < G; ++i) {
@@ -406,7 +385,8 @@ __device__ __forceinline__ void load_any(TileRegs<C, J>& r, const typename C::In
 }
 
 template <class C, int NW, int J, int SEG, int LOC>
-__device__ __forceinline__ void output_any(TileRegs<C, J>& r, const PipeSlot<C, NW>& sl,
+__device__ __forceinline__ void output_any(TileRegs<C, J>& r, const typename C::Acc (&gex)[J],
+                                           const int (&gexf)[J], const PipeSlot<C, NW>& sl,
                                            typename C::Out* __restrict__ y, const ScanArgs& a,
                                            uint32_t tile, int wid, int lane,
                                            uint16_t* __restrict__ stage) {
@@ -418,111 +398,99 @@ __device__ __forceinline__ void output_any(TileRegs<C, J>& r, const PipeSlot<C, 
       tile_output_mfma<C, NW, J, false>(r, sl, y, a.n, tile, wid, lane, stage);
     }
   } else if (full) {
-    tile_output<C, NW, J, SEG, true>(r, sl, y, a.n, tile, wid, lane);
+    tile_output<C, NW, J, SEG, true>(r, gex, gexf, sl, y, a.n, tile, wid, lane);
   } else {
-    tile_output<C, NW, J, SEG, false>(r, sl, y, a.n, tile, wid, lane);
+    tile_output<C, NW, J, SEG, false>(r, gex, gexf, sl, y, a.n, tile, wid, lane);
   }
 }
 
 // Persistent pipelined scan.  Workgroups draw tiles from ticket counters in
-// the workspace header.  A single counter serialises every draw on one
-// address (about 12 ns each on MI355X, which caps small tiles), so there are
-// up to kTicketCounters of them: workgroup b belongs to group k = b % K and
-// counter k hands out the tiles k, k + K, k + 2K, ... (consecutive
-// workgroups land on different XCDs, so each XCD mostly draws from its own
-// counter).
-//
-// Each workgroup runs a three-stage software pipeline over the tiles it
-// holds: in one step the data waves issue the loads of tile t + 2, reduce
-// tile t + 1 (loaded during the previous step, so its load latency is
-// hidden and its loads were issued before the stores of tile t - 1, which
-// they therefore never wait for) and publish its aggregate, while the
-// look-back wave resolves tile t's carry and draws the next ticket; after
-// one barrier the data waves add t's carry and store t.
-//
-// Every tile below the smallest undrawn one is held by a running workgroup
-// or done, and a held tile's aggregate is published within two steps of its
-// workgroup, whose own waits only involve smaller tiles; so the smallest
-// unfinished tile can always finish and every walk terminates (the
-// look-back's fallback covers workgroups that are not resident).  A
-// workgroup stops drawing after its first draw past its group's last tile,
-// so counter k sees exactly tiles_k + wgs_k draws; the last one resets it
-// for the next launch on the workspace.
+// the workspace header, one tile ahead.  A single counter serialises every
+// draw on one address (about 12 ns each on MI355X, which caps small tiles),
+// so there are up to kTicketCounters of them: workgroup b belongs to group
+// k = b % K and counter k hands out the tiles k, k + K, k + 2K, ...
+// (consecutive workgroups land on different XCDs, so each XCD mostly draws
+// from its own counter).  A workgroup only holds tiles while it runs and
+// publishes each held tile's aggregate as soon as the tile is reduced;
+// every tile below the smallest undrawn one is held by a running workgroup
+// or done, so the smallest unfinished tile can always finish and every walk
+// terminates (the look-back's fallback covers workgroups that are not
+// resident).  A workgroup stops drawing after its first draw past its
+// group's last tile, so counter k sees exactly tiles_k + wgs_k draws; the
+// last one resets it for the next launch on the workspace.
 template <class C, int NW, int J, int SEG, int LOC>
 __global__ __launch_bounds__((NW + 1) * kWave) void scan_kernel(
     const typename C::In* __restrict__ x, typename C::Out* __restrict__ y,
     const int8_t* __restrict__ flags, typename C::Acc start, void* ws, ScanArgs a) {
+  using Acc = typename C::Acc;
   constexpr int kStage = LOC == kLocMfma ? UnitScan<C>::kStageHalfs : 1;
-  __shared__ PipeSlot<C, NW> slot[3];
-  __shared__ uint32_t s_draw[3];
+  __shared__ PipeSlot<C, NW> slot[2];
+  __shared__ uint32_t s_draw[2];
   __shared__ uint32_t s_tag;
   __shared__ __attribute__((aligned(16))) uint16_t stage_all[NW][kStage];
   const int lane = __lane_id();
   const int wid = threadIdx.x / kWave;  // wave NW is the look-back wave
   const bool lb_wave = wid == NW;
-  const uint32_t NT = a.num_tiles;
   // Ticket group of this workgroup; K <= num_tiles (the grid never exceeds
   // the tile count), so every group has at least one tile and one workgroup.
   const uint32_t K = gridDim.x < (uint32_t)kTicketCounters ? gridDim.x : (uint32_t)kTicketCounters;
   const uint32_t grp = blockIdx.x % K;
   uint64_t* ctr = ws_slot(ws, (int)grp);
-  const uint64_t grp_tiles = ((uint64_t)NT - grp + K - 1) / K;
+  const uint64_t grp_tiles = ((uint64_t)a.num_tiles - grp + K - 1) / K;
   const uint64_t grp_wgs = (gridDim.x - grp + K - 1) / K;
   const uint64_t last_draw = grp_tiles + grp_wgs - 1;
   auto draw = [&]() -> uint32_t {
     const uint64_t v = __hip_atomic_fetch_add(ctr, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (v == last_draw) __hip_atomic_store(ctr, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    return v >= grp_tiles ? NT : (uint32_t)(v * K + grp);
+    return v >= grp_tiles ? a.num_tiles : (uint32_t)(v * K + grp);
   };
 
-  if (threadIdx.x < 3) slot[threadIdx.x].count = 0;
+  if (threadIdx.x < 2) {
+    slot[threadIdx.x].count = 0;
+  }
   if (lb_wave && lane == 0) {
     s_tag = launch_arrive(ws);
     const uint32_t d0 = draw();
-    const uint32_t d1 = d0 < NT ? draw() : NT;
     s_draw[0] = d0;
-    s_draw[1] = d1;
-    s_draw[2] = d1 < NT ? draw() : NT;
+    s_draw[1] = d0 < a.num_tiles ? draw() : a.num_tiles;
   }
   __syncthreads();
   a.tag = s_tag;
   uint32_t t = s_draw[0];
   uint32_t tn = s_draw[1];
-  uint32_t tnn = s_draw[2];
-  if (t >= NT) return;
+  if (t >= a.num_tiles) return;
 
-  TileRegs<C, J> reg[3];
+  TileRegs<C, J> reg[2];
+  Acc gex[2][J];
+  int gexf[2][J];
   uint16_t* stage = stage_all[lb_wave ? 0 : wid];
   if (!lb_wave) {
     load_any<C, NW, J, SEG, LOC>(reg[0], x, flags, a, t, wid, lane);
-    if (tn < NT) load_any<C, NW, J, SEG, LOC>(reg[1], x, flags, a, tn, wid, lane);
-    tile_reduce<C, NW, J, SEG, LOC>(reg[0], slot[0], start, ws, a, t, wid, lane);
+    tile_reduce<C, NW, J, SEG, LOC>(reg[0], gex[0], gexf[0], slot[0], start, ws, a, t, wid, lane);
   }
   __syncthreads();
 
-  // Step with tile t in stage P, tn (loads in flight) in P + 1 and tnn (to
-  // load now) in P + 2, all mod 3.  Returns false when no tile is left.
-  auto step = [&](auto stage_idx) -> bool {
-    constexpr int P = decltype(stage_idx)::value;
-    constexpr int P1 = (P + 1) % 3, P2 = (P + 2) % 3;
+  // One pipeline step with tile t in parity P and the next tile tn going
+  // into parity 1 - P.  Returns false when the workgroup has no next tile.
+  auto step = [&](auto parity) -> bool {
+    constexpr int P = decltype(parity)::value;
+    constexpr int Q = 1 - P;
     if (lb_wave) {
       tile_lookback<C, NW, J, SEG>(slot[P], x, flags, start, ws, a, t, lane);
-      if (lane == 0) s_draw[P] = tnn < NT ? draw() : NT;
-    } else {
-      if (tnn < NT) load_any<C, NW, J, SEG, LOC>(reg[P2], x, flags, a, tnn, wid, lane);
-      if (tn < NT) tile_reduce<C, NW, J, SEG, LOC>(reg[P1], slot[P1], start, ws, a, tn, wid, lane);
+      if (lane == 0) s_draw[P] = tn < a.num_tiles ? draw() : a.num_tiles;
+    } else if (tn < a.num_tiles) {
+      load_any<C, NW, J, SEG, LOC>(reg[Q], x, flags, a, tn, wid, lane);
+      tile_reduce<C, NW, J, SEG, LOC>(reg[Q], gex[Q], gexf[Q], slot[Q], start, ws, a, tn, wid, lane);
     }
     __syncthreads();
-    if (!lb_wave) output_any<C, NW, J, SEG, LOC>(reg[P], slot[P], y, a, t, wid, lane, stage);
+    if (!lb_wave) output_any<C, NW, J, SEG, LOC>(reg[P], gex[P], gexf[P], slot[P], y, a, t, wid, lane, stage);
     t = tn;
-    tn = tnn;
-    tnn = s_draw[P];
-    return t < NT;
+    tn = s_draw[P];
+    return t < a.num_tiles;
   };
   for (;;) {
     if (!step(std::integral_constant<int, 0>{})) break;
     if (!step(std::integral_constant<int, 1>{})) break;
-    if (!step(std::integral_constant<int, 2>{})) break;
   }
 }
 
@@ -602,8 +570,7 @@ static int scan_dispatch(const void* x, int x_dtype, void* y, const int8_t* flag
                          size_t ws_bytes, void* stream, int method = TCS_METHOD_AUTO,
                          int window = 0) {
   if (n < 0 || S <= 0) return TCS_ERR_INVALID_ARG;
-  if (window == 0) window = kLookbackWindow;
-  if (window < 1 || window > kWave) return TCS_ERR_INVALID_ARG;
+  if (window < 0 || window > kWave) return TCS_ERR_INVALID_ARG;
   if (method != TCS_METHOD_AUTO && method != TCS_METHOD_DPP && method != TCS_METHOD_MFMA)
     return TCS_ERR_INVALID_ARG;
   if (method == TCS_METHOD_MFMA && (SEG != kSegNone || (x_dtype != TCS_F16 && x_dtype != TCS_I8)))
@@ -612,6 +579,10 @@ static int scan_dispatch(const void* x, int x_dtype, void* y, const int8_t* flag
   // MI355X at N = 2^28: the matrix-unit scan for int8 (4.1 vs 3.5 TB/s),
   // the DPP scan otherwise (fp16: 4.8 vs 3.5 TB/s).
   if (method == TCS_METHOD_AUTO) method = (SEG == kSegNone && x_dtype == TCS_I8) ? TCS_METHOD_MFMA : TCS_METHOD_DPP;
+  // Default look-back window, measured at N = 2^28 with the default tiles:
+  // 8 for the DPP scans (fp16: 4.96 vs 4.78 TB/s at 16), 32 for the int8
+  // matrix-unit scan (4.40 vs 4.09 TB/s at 16).
+  if (window == 0) window = method == TCS_METHOD_MFMA ? 32 : 8;
   if (n == 0) return TCS_OK;
   if (!x || !y || (SEG == kSegFlags && !flags)) return TCS_ERR_INVALID_ARG;
   if (SEG == kSegFixed && seg_len <= 0) return TCS_ERR_INVALID_ARG;

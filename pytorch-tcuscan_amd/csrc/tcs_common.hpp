@@ -164,18 +164,22 @@ __device__ __forceinline__ uint64_t* ws_desc(void* ws) {
 // kernel argument would be frozen into the graph).  Each workgroup then
 // arrives on its group's counter (group = blockIdx.x % K); the last arrival
 // of a group arrives on the groups counter, and the last group advances
-// the epoch for the next launch.  Every workgroup has read the epoch before
-// it arrives (release), so the advance happens after all reads.
+// the epoch for the next launch.  Every workgroup's epoch read has returned
+// before its arrival is issued (the wait below), so the advance comes after
+// all reads.  The atomics are relaxed: acquire/release at agent scope would
+// write back and invalidate L2 in every workgroup, which costs streaming
+// kernels with thousands of workgroups several times their run time.
 __device__ __forceinline__ uint32_t launch_arrive(void* ws) {
   const uint64_t e = __hip_atomic_load(ws_slot(ws, kSlotEpoch), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
   const uint32_t K = gridDim.x < (uint32_t)kTicketCounters ? gridDim.x : (uint32_t)kTicketCounters;
   const uint32_t g = blockIdx.x % K;
   const uint64_t members = (gridDim.x - g + K - 1) / K;
   uint64_t* c = ws_slot(ws, kSlotArrived0 + (int)g);
-  if (__hip_atomic_fetch_add(c, 1ull, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) == members - 1) {
+  if (__hip_atomic_fetch_add(c, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == members - 1) {
     __hip_atomic_store(c, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     uint64_t* d = ws_slot(ws, kSlotGroupsArrived);
-    if (__hip_atomic_fetch_add(d, 1ull, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) == K - 1) {
+    if (__hip_atomic_fetch_add(d, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == K - 1) {
       __hip_atomic_store(d, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       __hip_atomic_store(ws_slot(ws, kSlotEpoch), e + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
