@@ -37,11 +37,18 @@ sys.path.insert(0, PKG)
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E peak
 
 
-def _timed_steps(fn, steps: int, warmup: int):
+def _timed_steps(fn, steps: int, warmup: int, spin_s: float = 0.25):
     """Runs fn warmup+steps times; returns (wall seconds for the timed steps,
-    per-step event milliseconds)."""
+    per-step event milliseconds).  Before the warm-up steps, fn runs for at
+    least `spin_s` seconds so the GPU is at its working clocks when timing
+    starts (a few short kernels after idle measure the clock ramp)."""
     import torch
 
+    t_end = time.perf_counter() + spin_s
+    while time.perf_counter() < t_end:
+        for _ in range(4):
+            fn()
+        torch.cuda.synchronize()
     for _ in range(warmup):
         fn()
     torch.cuda.synchronize()
