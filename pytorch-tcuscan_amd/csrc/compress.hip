@@ -4,8 +4,9 @@
 // CompareScalar EQ 1, src/kernels/kernel_compress.h:167-168), in order.
 // One pass: each tile counts its kept elements (lane-serial, then DPP wave
 // scans and a block scan), the tile's output offset arrives through the
-// decoupled look-back on counts, and every lane writes its kept elements at
-// consecutive positions.  Optionally the kept positions (int32) or the
+// decoupled look-back on counts, each wave stages its kept elements in LDS
+// in output order and writes its contiguous output range with coalesced
+// stores (lane k of a store writes element k of a 64-element run).  Optionally the kept positions (int32) or the
 // matching entries of an index array are written beside the values
 // (run_compress_ind / _no_arange).  A separate count pass sizes the outputs
 // when the caller does not know the count (the reference syncs on a reduce
@@ -16,6 +17,8 @@
 // int16: two's-complement wrap), following
 // src/kernels/kernel_csr_gather.h:122-137.  Columns outside [0, x_len)
 // yield 0 rather than an out-of-bounds read.
+#include <type_traits>
+
 #include "tcs_common.hpp"
 
 namespace tcs {
@@ -40,9 +43,11 @@ __global__ __launch_bounds__(kCmpNW * kWave) void compress_kernel(
   using IT = typename Bits<IB>::T;
   constexpr int G = kCmpG, J = kCmpJ, NW = kCmpNW;
   constexpr int64_t GSPAN = (int64_t)kWave * G, WSEG = GSPAN * J;
+  using ST = typename Bits<(XB > IB ? XB : IB)>::T;
   __shared__ int s_wval[NW], s_wexcl[NW];
   __shared__ int64_t s_base;
   __shared__ uint32_t s_tag;
+  __shared__ ST s_stage[NW][J * G * kWave];  // one wave's kept elements, in order
 
   if (threadIdx.x == 0) s_tag = launch_arrive(ws);
   const int lane = __lane_id();
@@ -111,23 +116,45 @@ __global__ __launch_bounds__(kCmpNW * kWave) void compress_kernel(
   }
   __syncthreads();
 
+  // This wave's kept elements go to [base, base + wtot), capped at cap.
   const int64_t base = s_base + s_wexcl[wid];
+  ST* stage = s_stage[wid];
+  const int64_t room = cap - base;
+  const int wn = room <= 0 ? 0 : (room < wtot ? (int)room : wtot);
+  auto flush = [&](auto* out) {
+    using OT = typename std::remove_pointer<decltype(out)>::type;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (int k = lane; k < wn; k += kWave) out[base + k] = (OT)stage[k];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  };
 #pragma unroll
   for (int j = 0; j < J; ++j) {
-    int64_t pos = base + gex[j];
-    const int64_t gb = wbase + j * GSPAN;
+    int pos = gex[j];
 #pragma unroll
-    for (int i = 0; i < G; ++i) {
-      if (m[j][i] == 1) {
-        if (pos < cap) {
-          out_x[pos] = xv[j][i];
-          if constexpr (IDX == 1) out_idx[pos] = iv[j][i];
-          if constexpr (IDX == 2) out_idx[pos] = (IT)(gb + i);
-          if constexpr (IDX == 3) out_idx[pos] = (IT)(gb + i + 1);
+    for (int i = 0; i < G; ++i)
+      if (m[j][i] == 1) stage[pos++] = (ST)xv[j][i];
+  }
+  flush(out_x);
+  if constexpr (IDX != 0) {
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+      int pos = gex[j];
+      const int64_t gb = wbase + j * GSPAN;
+#pragma unroll
+      for (int i = 0; i < G; ++i) {
+        if (m[j][i] == 1) {
+          if constexpr (IDX == 1) stage[pos] = (ST)iv[j][i];
+          if constexpr (IDX == 2) stage[pos] = (ST)(IT)(gb + i);
+          if constexpr (IDX == 3) stage[pos] = (ST)(IT)(gb + i + 1);
+          ++pos;
         }
-        ++pos;
       }
     }
+    flush(out_idx);
   }
 }
 
