@@ -293,12 +293,23 @@ __device__ __forceinline__ void tile_lookback(PipeSlot<C, NW>& sl,
   if (lane == 0) sl.carry = carry;
 }
 
+// Per-wave LDS staging of one group's outputs: with 8-wide lane groups a
+// lane's 8 outputs are 32 contiguous bytes, so direct 16-byte stores leave
+// every store instruction writing half of each 32-byte span; staged, each
+// store instruction writes 1 KiB contiguously.
+template <class C, int LOC>
+struct OutStage {
+  static constexpr bool kOn = LOC == kLocDpp && C::G == 8 && sizeof(typename C::Out) == 4;
+  static constexpr int kWords = kOn ? kWave * C::G : 1;
+};
+
 // Second pass over the tile: add the carries and store.
 template <class C, int NW, int J, int SEG, bool FULL>
 __device__ __forceinline__ void tile_output(TileRegs<C, J>& r, const typename C::Acc (&gex)[J],
                                             const int (&gexf)[J], const PipeSlot<C, NW>& sl,
                                             typename C::Out* __restrict__ y, int64_t n,
-                                            uint32_t tile, int wid, int lane) {
+                                            uint32_t tile, int wid, int lane,
+                                            uint32_t* __restrict__ os) {
   using Acc = typename C::Acc;
   using Out = typename C::Out;
   using Gm = ScanGeom<C, NW, J>;
@@ -328,7 +339,26 @@ __device__ __forceinline__ void tile_output(TileRegs<C, J>& r, const typename C:
       }
       o[i] = C::out(run);
     }
-    store_items<Out, G, FULL>(y, wbase + j * Gm::GSPAN, n, o);
+    if constexpr (FULL && OutStage<C, kLocDpp>::kOn) {
+      u32x4 v0, v1;
+      __builtin_memcpy(&v0, &o[0], 16);
+      __builtin_memcpy(&v1, &o[4], 16);
+      *reinterpret_cast<u32x4*>(os + lane * 8) = v0;
+      *reinterpret_cast<u32x4*>(os + lane * 8 + 4) = v1;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      const u32x4 w0 = *reinterpret_cast<const u32x4*>(os + lane * 4);
+      const u32x4 w1 = *reinterpret_cast<const u32x4*>(os + kWave * 4 + lane * 4);
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      Out* gy = y + (wbase - (int64_t)lane * G) + j * Gm::GSPAN;
+      *reinterpret_cast<u32x4*>(gy + lane * 4) = w0;
+      *reinterpret_cast<u32x4*>(gy + kWave * 4 + lane * 4) = w1;
+    } else {
+      store_items<Out, G, FULL>(y, wbase + j * Gm::GSPAN, n, o);
+    }
   }
 }
 
@@ -389,7 +419,7 @@ __device__ __forceinline__ void output_any(TileRegs<C, J>& r, const typename C::
                                            const int (&gexf)[J], const PipeSlot<C, NW>& sl,
                                            typename C::Out* __restrict__ y, const ScanArgs& a,
                                            uint32_t tile, int wid, int lane,
-                                           uint16_t* __restrict__ stage) {
+                                           uint16_t* __restrict__ stage, uint32_t* __restrict__ os) {
   const bool full = (int64_t)(tile + 1) * ScanGeom<C, NW, J>::TILE <= a.n;
   if constexpr (LOC == kLocMfma) {
     if (full) {
@@ -398,9 +428,9 @@ __device__ __forceinline__ void output_any(TileRegs<C, J>& r, const typename C::
       tile_output_mfma<C, NW, J, false>(r, sl, y, a.n, tile, wid, lane, stage);
     }
   } else if (full) {
-    tile_output<C, NW, J, SEG, true>(r, gex, gexf, sl, y, a.n, tile, wid, lane);
+    tile_output<C, NW, J, SEG, true>(r, gex, gexf, sl, y, a.n, tile, wid, lane, os);
   } else {
-    tile_output<C, NW, J, SEG, false>(r, gex, gexf, sl, y, a.n, tile, wid, lane);
+    tile_output<C, NW, J, SEG, false>(r, gex, gexf, sl, y, a.n, tile, wid, lane, os);
   }
 }
 
@@ -428,6 +458,7 @@ __global__ __launch_bounds__((NW + 1) * kWave) void scan_kernel(
   __shared__ uint32_t s_draw[2];
   __shared__ uint32_t s_tag;
   __shared__ __attribute__((aligned(16))) uint16_t stage_all[NW][kStage];
+  __shared__ __attribute__((aligned(16))) uint32_t ostage_all[NW][OutStage<C, LOC>::kWords];
   const int lane = __lane_id();
   const int wid = threadIdx.x / kWave;  // wave NW is the look-back wave
   const bool lb_wave = wid == NW;
@@ -464,6 +495,7 @@ __global__ __launch_bounds__((NW + 1) * kWave) void scan_kernel(
   Acc gex[2][J];
   int gexf[2][J];
   uint16_t* stage = stage_all[lb_wave ? 0 : wid];
+  uint32_t* ostage = ostage_all[lb_wave ? 0 : wid];
   if (!lb_wave) {
     load_any<C, NW, J, SEG, LOC>(reg[0], x, flags, a, t, wid, lane);
     tile_reduce<C, NW, J, SEG, LOC>(reg[0], gex[0], gexf[0], slot[0], start, ws, a, t, wid, lane);
@@ -483,7 +515,7 @@ __global__ __launch_bounds__((NW + 1) * kWave) void scan_kernel(
       tile_reduce<C, NW, J, SEG, LOC>(reg[Q], gex[Q], gexf[Q], slot[Q], start, ws, a, tn, wid, lane);
     }
     __syncthreads();
-    if (!lb_wave) output_any<C, NW, J, SEG, LOC>(reg[P], gex[P], gexf[P], slot[P], y, a, t, wid, lane, stage);
+    if (!lb_wave) output_any<C, NW, J, SEG, LOC>(reg[P], gex[P], gexf[P], slot[P], y, a, t, wid, lane, stage, ostage);
     t = tn;
     tn = s_draw[P];
     return t < a.num_tiles;
