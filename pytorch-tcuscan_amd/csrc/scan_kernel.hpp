@@ -162,9 +162,20 @@ __device__ __forceinline__ void tile_load(TileRegs<C, J>& r, const typename C::I
                                           uint32_t tile, int wid, int lane) {
   using Gm = ScanGeom<C, NW, J>;
   const int64_t wave0 = (int64_t)tile * Gm::TILE + (int64_t)wid * Gm::WSEG;
+  if constexpr (LOC == kLocMfma && sizeof(typename C::In) == 1 && FULL) {
+    // The two 8-byte pieces of an int8 unit are adjacent: one 16-byte load
+    // (a wave instruction then covers 1 KiB contiguously).
 #pragma unroll
-  for (int j = 0; j < J; ++j)
-    load_items<typename C::In, Gm::G, FULL>(x, wave0 + group_offset<C, LOC>(j, lane), a.n, r.raw[j]);
+    for (int u = 0; u < J / 2; ++u) {
+      const u32x4 v = *reinterpret_cast<const u32x4*>(x + wave0 + (int64_t)kUnit * u + UnitScan<C>::piece(0, lane));
+      __builtin_memcpy(r.raw[2 * u], &v, 8);
+      __builtin_memcpy(r.raw[2 * u + 1], reinterpret_cast<const char*>(&v) + 8, 8);
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < J; ++j)
+      load_items<typename C::In, Gm::G, FULL>(x, wave0 + group_offset<C, LOC>(j, lane), a.n, r.raw[j]);
+  }
   const int64_t wbase = wave0 + (int64_t)lane * Gm::G;
 #pragma unroll
   for (int j = 0; j < J; ++j)
@@ -297,10 +308,12 @@ __device__ __forceinline__ void tile_lookback(PipeSlot<C, NW>& sl,
 // lane's 8 outputs are 32 contiguous bytes, so direct 16-byte stores leave
 // every store instruction writing half of each 32-byte span; staged, each
 // store instruction writes 1 KiB contiguously.
+// The matrix-unit scan stages a whole 1024-element unit: its lanes hold
+// runs of 4 outputs 32 elements apart.
 template <class C, int LOC>
 struct OutStage {
-  static constexpr bool kOn = LOC == kLocDpp && C::G == 8 && sizeof(typename C::Out) == 4;
-  static constexpr int kWords = kOn ? kWave * C::G : 1;
+  static constexpr bool kOn = sizeof(typename C::Out) == 4 && (LOC == kLocMfma || C::G == 8);
+  static constexpr int kWords = !kOn ? 1 : (LOC == kLocMfma ? kUnit : kWave * C::G);
 };
 
 // Second pass over the tile: add the carries and store.
@@ -370,7 +383,8 @@ template <class C, int NW, int J, bool FULL>
 __device__ __forceinline__ void tile_output_mfma(TileRegs<C, J>& r, const PipeSlot<C, NW>& sl,
                                                  typename C::Out* __restrict__ y, int64_t n,
                                                  uint32_t tile, int wid, int lane,
-                                                 uint16_t* __restrict__ stage) {
+                                                 uint16_t* __restrict__ stage,
+                                                 uint32_t* __restrict__ os) {
   using Acc = typename C::Acc;
   using Out = typename C::Out;
   using Gm = ScanGeom<C, NW, J>;
@@ -392,12 +406,35 @@ __device__ __forceinline__ void tile_output_mfma(TileRegs<C, J>& r, const PipeSl
     const Acc sh = dpp_mov<kWaveShr1>(incl);
     const Acc base = run + (row == 0 ? Acc(0) : sh);
     const int64_t ub = wave0 + (int64_t)kUnit * u;
+    if constexpr (FULL) {
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      Out o[4];
+      for (int q = 0; q < 4; ++q) {
+        Out o[4];
 #pragma unroll
-      for (int s = 0; s < 4; ++s) o[s] = C::out(base + v[4 * q + s]);
-      store_items<Out, 4, FULL>(y, ub + unit_out_pos(q, lane), n, o);
+        for (int s = 0; s < 4; ++s) o[s] = C::out(base + v[4 * q + s]);
+        u32x4 w;
+        __builtin_memcpy(&w, o, 16);
+        *reinterpret_cast<u32x4*>(os + unit_out_pos(q, lane)) = w;
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      u32x4 w[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) w[k] = *reinterpret_cast<const u32x4*>(os + k * kWave * 4 + lane * 4);
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+      for (int k = 0; k < 4; ++k) *reinterpret_cast<u32x4*>(y + ub + k * kWave * 4 + lane * 4) = w[k];
+    } else {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        Out o[4];
+#pragma unroll
+        for (int s = 0; s < 4; ++s) o[s] = C::out(base + v[4 * q + s]);
+        store_items<Out, 4, FULL>(y, ub + unit_out_pos(q, lane), n, o);
+      }
     }
     run = run + readlane(incl, 31);
   }
@@ -423,9 +460,9 @@ __device__ __forceinline__ void output_any(TileRegs<C, J>& r, const typename C::
   const bool full = (int64_t)(tile + 1) * ScanGeom<C, NW, J>::TILE <= a.n;
   if constexpr (LOC == kLocMfma) {
     if (full) {
-      tile_output_mfma<C, NW, J, true>(r, sl, y, a.n, tile, wid, lane, stage);
+      tile_output_mfma<C, NW, J, true>(r, sl, y, a.n, tile, wid, lane, stage, os);
     } else {
-      tile_output_mfma<C, NW, J, false>(r, sl, y, a.n, tile, wid, lane, stage);
+      tile_output_mfma<C, NW, J, false>(r, sl, y, a.n, tile, wid, lane, stage, os);
     }
   } else if (full) {
     tile_output<C, NW, J, SEG, true>(r, gex, gexf, sl, y, a.n, tile, wid, lane, os);
