@@ -85,8 +85,8 @@ int tcs_scan(const void* x, int x_dtype, void* y, int64_t n, int S, int tiles_pe
 
 /* Local scan method of tcs_scan_method. */
 enum tcs_scan_method {
-  TCS_METHOD_AUTO = 0, /* the shipped method, the faster one measured on MI355X:
-                          MFMA for plain int8 scans, DPP otherwise */
+  TCS_METHOD_AUTO = 0, /* the shipped method, the faster one measured on MI355X
+                          (DPP for every dtype) */
   TCS_METHOD_DPP = 1,  /* lane-serial group sums joined by DPP wavefront scans */
   TCS_METHOD_MFMA = 2  /* 32x32 units scanned on the matrix cores (F16, I8 only) */
 };

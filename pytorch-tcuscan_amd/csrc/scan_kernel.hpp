@@ -48,7 +48,7 @@ struct ScanGeom {
 // Per-lane registers of one tile held by a data wave.
 template <class C, int J>
 struct TileRegs {
-  typename C::In raw[J][C::G];
+  Packed<typename C::In, C::G> raw[J];
   uint32_t heads[J];
 };
 
@@ -150,7 +150,8 @@ struct ScanArgs {
 template <class C, int LOC>
 __device__ __forceinline__ int64_t group_offset(int j, int lane) {
   if constexpr (LOC == kLocMfma) {
-    return (int64_t)kUnit * (j >> 1) + UnitScan<C>::piece(j & 1, lane);
+    constexpr int kG = UnitScan<C>::kGroups;
+    return (int64_t)kUnit * (j / kG) + UnitScan<C>::piece(j % kG, lane);
   } else {
     return (int64_t)j * kWave * C::G + (int64_t)lane * C::G;
   }
@@ -162,20 +163,9 @@ __device__ __forceinline__ void tile_load(TileRegs<C, J>& r, const typename C::I
                                           uint32_t tile, int wid, int lane) {
   using Gm = ScanGeom<C, NW, J>;
   const int64_t wave0 = (int64_t)tile * Gm::TILE + (int64_t)wid * Gm::WSEG;
-  if constexpr (LOC == kLocMfma && sizeof(typename C::In) == 1 && FULL) {
-    // The two 8-byte pieces of an int8 unit are adjacent: one 16-byte load
-    // (a wave instruction then covers 1 KiB contiguously).
 #pragma unroll
-    for (int u = 0; u < J / 2; ++u) {
-      const u32x4 v = *reinterpret_cast<const u32x4*>(x + wave0 + (int64_t)kUnit * u + UnitScan<C>::piece(0, lane));
-      __builtin_memcpy(r.raw[2 * u], &v, 8);
-      __builtin_memcpy(r.raw[2 * u + 1], reinterpret_cast<const char*>(&v) + 8, 8);
-    }
-  } else {
-#pragma unroll
-    for (int j = 0; j < J; ++j)
-      load_items<typename C::In, Gm::G, FULL>(x, wave0 + group_offset<C, LOC>(j, lane), a.n, r.raw[j]);
-  }
+  for (int j = 0; j < J; ++j)
+    load_items<typename C::In, Gm::G, FULL>(x, wave0 + group_offset<C, LOC>(j, lane), a.n, r.raw[j]);
   const int64_t wbase = wave0 + (int64_t)lane * Gm::G;
 #pragma unroll
   for (int j = 0; j < J; ++j)
@@ -388,16 +378,21 @@ __device__ __forceinline__ void tile_output_mfma(TileRegs<C, J>& r, const PipeSl
   using Acc = typename C::Acc;
   using Out = typename C::Out;
   using Gm = ScanGeom<C, NW, J>;
-  static_assert(J % 2 == 0, "units are pairs of groups");
+  constexpr int kG = UnitScan<C>::kGroups;
+  static_assert(J % kG == 0, "a unit is kG lane groups");
 #pragma unroll
   for (int j = 0; j < J; ++j) reg_barrier(r.raw[j]);
   Acc run = sl.carry + sl.wexcl[wid];
   const int64_t wave0 = (int64_t)tile * Gm::TILE + (int64_t)wid * Gm::WSEG;
   const int row = lane & 31;
 #pragma unroll
-  for (int u = 0; u < J / 2; ++u) {
+  for (int u = 0; u < J / kG; ++u) {
     Acc v[16];
-    UnitScan<C>::scan(r.raw[2 * u], r.raw[2 * u + 1], v, stage, lane);
+    if constexpr (kG == 2) {
+      UnitScan<C>::scan(r.raw[2 * u], r.raw[2 * u + 1], v, stage, lane);
+    } else {
+      UnitScan<C>::scan(r.raw[u], v, stage, lane);
+    }
     // Row total of row `row`: accumulator 15 of lane row + 32 (m = 31).
     const Acc tot = __shfl(v[15], row + 32);
     const Acc incl = half_wave_incl_scan(tot);
@@ -437,6 +432,9 @@ __device__ __forceinline__ void tile_output_mfma(TileRegs<C, J>& r, const PipeSl
       }
     }
     run = run + readlane(incl, 31);
+    // Keep each unit's product, offsets and stores together: hoisting the
+    // next units' MFMAs above them would hold their 16 accumulators live.
+    __builtin_amdgcn_sched_barrier(0);
   }
 }
 
@@ -601,7 +599,18 @@ static int launch_scan_tile(int64_t tile, const void* x, void* y, const int8_t* 
     hipLaunchKernelGGL((scan_kernel<C, NW, J, SEG, LOC>), dim3(grid), dim3((NW + 1) * kWave), 0, \
                        st, xi, yo, flags, start, ws, ScanArgs{n, seg_len, tiles, 0, window});  \
   } while (0)
-  if constexpr (LOC == kLocMfma) {
+  if constexpr (LOC == kLocMfma && UnitScan<C>::kGroups == 1) {
+    switch (groups) {
+      case 1: TCS_LAUNCH(1, 1); break;
+      case 2: TCS_LAUNCH(2, 1); break;
+      case 4: TCS_LAUNCH(4, 1); break;
+      case 8: TCS_LAUNCH(4, 2); break;
+      case 16: TCS_LAUNCH(4, 4); break;
+      case 32: TCS_LAUNCH(8, 4); break;
+      case 64: TCS_LAUNCH(8, 8); break;
+      default: return TCS_ERR_INVALID_ARG;
+    }
+  } else if constexpr (LOC == kLocMfma) {
     switch (groups) {
       case 2: TCS_LAUNCH(1, 2); break;
       case 4: TCS_LAUNCH(2, 2); break;
@@ -644,28 +653,28 @@ static int scan_dispatch(const void* x, int x_dtype, void* y, const int8_t* flag
     return TCS_ERR_INVALID_ARG;
   if (method == TCS_METHOD_MFMA && (SEG != kSegNone || (x_dtype != TCS_F16 && x_dtype != TCS_I8)))
     return TCS_ERR_UNSUPPORTED_DTYPE;
-  // The shipped local scan of each dtype is the faster one measured on
-  // MI355X at N = 2^28: the matrix-unit scan for int8 (4.1 vs 3.5 TB/s),
-  // the DPP scan otherwise (fp16: 4.8 vs 3.5 TB/s).
-  if (method == TCS_METHOD_AUTO) method = (SEG == kSegNone && x_dtype == TCS_I8) ? TCS_METHOD_MFMA : TCS_METHOD_DPP;
-  // Default look-back window, measured at N = 2^28 with the default tiles:
-  // 8 for the DPP scans (fp16: 4.96 vs 4.78 TB/s at 16), 32 for the int8
-  // matrix-unit scan (4.40 vs 4.09 TB/s at 16).
+  // The shipped local scan is the faster one measured on MI355X at
+  // N = 2^28 with the default tiles: DPP for every dtype (fp16 5.6 vs 3.7
+  // TB/s on the matrix units; int8 5.1 vs 4.4).
+  if (method == TCS_METHOD_AUTO) method = TCS_METHOD_DPP;
+  // Default look-back window, measured the same way: 8 for DPP (int8: 5.09
+  // vs 4.87 TB/s at 16), 32 for the matrix-unit scan.
   if (window == 0) window = method == TCS_METHOD_MFMA ? 32 : 8;
   if (n == 0) return TCS_OK;
   if (!x || !y || (SEG == kSegFlags && !flags)) return TCS_ERR_INVALID_ARG;
   if (SEG == kSegFixed && seg_len <= 0) return TCS_ERR_INVALID_ARG;
   if (!aligned16(x) || !aligned16(y) || (flags && !aligned16(flags))) return TCS_ERR_INVALID_ARG;
-  // Default tile (tiles_per_wg <= 0): 32K elements for 16-bit inputs, 16K
-  // otherwise (measured best on MI355X; int8 at 32K spills registers).  At
-  // most 64 lane groups per tile: 32K elements for 8-wide groups, 16K for
-  // 4-wide (fp32/int32) groups.
-  // The matrix-unit flavour defaults to 16K fp16 (at 32K its double-
-  // buffered tiles spill) and 32K int8.
+  // Default tile (tiles_per_wg <= 0), measured best on MI355X: 32K elements
+  // for 16-bit and 8-bit inputs, 16K for 32-bit inputs (the 64-group
+  // maximum of 4-wide groups); the matrix-unit flavour uses 16K for fp16 (at
+  // 32K its double-buffered tiles spill) and 32K for int8.
   int64_t tile = tiles_per_wg > 0 ? tile_elems_for(S, tiles_per_wg)
-                 : method == TCS_METHOD_MFMA ? (x_dtype == TCS_F16 ? 16384 : 32768)
-                 : ((x_dtype == TCS_F16 || x_dtype == TCS_BF16) ? 32768 : 16384);
-  const int g = (x_dtype == TCS_F32 || x_dtype == TCS_I32) ? 4 : 8;
+                 : (method == TCS_METHOD_MFMA && x_dtype == TCS_F16) ? 16384
+                 : (x_dtype == TCS_F32 || x_dtype == TCS_I32) ? 16384 : 32768;
+  // Lane group width: 4 for 32-bit inputs, 8 for 16-bit and int8, 16 for
+  // int8 on the matrix-unit scan (one group per 1024-element unit).
+  const int g = (x_dtype == TCS_F32 || x_dtype == TCS_I32) ? 4
+                : (method == TCS_METHOD_MFMA && x_dtype == TCS_I8) ? 16 : 8;
   if (tile > 64 * kWave * g) tile = 64 * kWave * g;
   const uint64_t tiles = (uint64_t)((n + tile - 1) / tile);
   if (tiles >= 0xFFFFFFFFull) return TCS_ERR_INVALID_ARG;

@@ -10,7 +10,7 @@ int launch_scan_mfma(int x_dtype, int64_t tile, const void* x, void* y, int64_t 
     return launch_scan_tile<CodecF16, kSegNone, kLocMfma>(tile, x, y, nullptr, n, 0, (double)(float)start,
                                                           ws, tiles, window, st);
   if (x_dtype == TCS_I8)
-    return launch_scan_tile<CodecI8, kSegNone, kLocMfma>(tile, x, y, nullptr, n, 0, (int)start, ws,
+    return launch_scan_tile<CodecI8U, kSegNone, kLocMfma>(tile, x, y, nullptr, n, 0, (int)start, ws,
                                                          tiles, window, st);
   return TCS_ERR_UNSUPPORTED_DTYPE;
 }

@@ -44,6 +44,14 @@ struct CodecI8 {
   __device__ static Acc conv(In v) { return (int)v; }
   __device__ static Out out(Acc a) { return a; }
 };
+// int8 for the matrix-unit scan: a lane group is a lane's whole share of a
+// 1024-element unit (16 bytes), so one wave instruction loads 1 KiB.
+struct CodecI8U {
+  static constexpr int G = 16;
+  using In = int8_t; using Acc = int; using Out = int;
+  __device__ static Acc conv(In v) { return (int)v; }
+  __device__ static Out out(Acc a) { return a; }
+};
 struct CodecI32 {
   static constexpr int G = 4;
   using In = int; using Acc = int; using Out = int;
@@ -116,6 +124,58 @@ __device__ __forceinline__ void store_items(T* __restrict__ p, int64_t base, int
   }
 }
 
+// N elements of a 1-, 2- or 4-byte type held as packed 32-bit words.  A
+// lane's inputs stay packed in VGPRs (an element array of int8 or 16-bit
+// values tends to take one VGPR per element) and elements are extracted
+// where they are used.
+template <class T, int N>
+struct Packed {
+  static_assert(sizeof(T) == 1 || sizeof(T) == 2 || sizeof(T) == 4, "element size");
+  static_assert((N * sizeof(T)) % 4 == 0, "whole words");
+  static constexpr int kW = (int)(N * sizeof(T) / 4);
+  uint32_t w[kW];
+  __device__ __forceinline__ T operator[](int i) const {
+    if constexpr (sizeof(T) == 4) {
+      return __builtin_bit_cast(T, w[i]);
+    } else if constexpr (sizeof(T) == 2) {
+      return __builtin_bit_cast(T, (uint16_t)(w[i >> 1] >> (16 * (i & 1))));
+    } else {
+      return __builtin_bit_cast(T, (uint8_t)(w[i >> 2] >> (8 * (i & 3))));
+    }
+  }
+};
+
+// load_items into packed words.
+template <class T, int ITEMS, bool FULL = false>
+__device__ __forceinline__ void load_items(const T* __restrict__ p, int64_t base, int64_t n,
+                                           Packed<T, ITEMS>& r) {
+  constexpr int kBytes = ITEMS * (int)sizeof(T);
+  if (FULL || base + ITEMS <= n) {
+    if constexpr (kBytes % 16 == 0) {
+      const u32x4* vp = reinterpret_cast<const u32x4*>(p + base);
+#pragma unroll
+      for (int k = 0; k < kBytes / 16; ++k) {
+        const u32x4 v = vp[k];
+        r.w[4 * k] = v[0];
+        r.w[4 * k + 1] = v[1];
+        r.w[4 * k + 2] = v[2];
+        r.w[4 * k + 3] = v[3];
+      }
+    } else if constexpr (kBytes == 8) {
+      const u32x2 v = *reinterpret_cast<const u32x2*>(p + base);
+      r.w[0] = v[0];
+      r.w[1] = v[1];
+    } else {
+      r.w[0] = *reinterpret_cast<const uint32_t*>(p + base);
+    }
+  } else {
+    T e[ITEMS];
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) e[i] = (base + i < n) ? p[base + i] : T(0);
+    __builtin_memcpy(r.w, e, kBytes);
+  }
+}
+
 // Marks the raw words of `r` as rewritten so the compiler recomputes values
 // derived from them instead of keeping those (wider) values live in VGPRs.
 template <class T, int ITEMS>
@@ -126,6 +186,12 @@ __device__ __forceinline__ void reg_barrier(T (&r)[ITEMS]) {
 #pragma unroll
   for (int k = 0; k < kWords; ++k) asm volatile("" : "+v"(w[k]));
   __builtin_memcpy(r, w, ITEMS * sizeof(T));
+}
+
+template <class T, int N>
+__device__ __forceinline__ void reg_barrier(Packed<T, N>& r) {
+#pragma unroll
+  for (int k = 0; k < Packed<T, N>::kW; ++k) asm volatile("" : "+v"(r.w[k]));
 }
 
 // ---------------------------------------------------------------------------

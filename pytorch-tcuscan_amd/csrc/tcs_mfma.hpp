@@ -23,7 +23,8 @@
 // products with the zero entries of L would not (inf * 0 = NaN).
 //
 // int8.  v_mfma_i32_32x32x32_i8 covers a whole row (K = 32) in one
-// instruction and accumulates exactly in int32.
+// instruction and accumulates exactly in int32; a lane's 16 bytes of a unit
+// are one lane group (CodecI8U).
 //
 // This replaces the reference's Cube-unit row scan C = A * U
 // (src/kernels/kernel_row_scan.h:97-131) in transposed form.
@@ -51,22 +52,24 @@ __device__ __forceinline__ int unit_out_pos(int q, int lane) {
 template <class C>
 struct UnitScan {
   static constexpr int kStageHalfs = 1;
+  static constexpr int kGroups = 2;  // lane groups per unit
 };
 
 template <>
 struct UnitScan<CodecF16> {
   static constexpr int kStageHalfs = kUnit;  // LDS staging per wave (sequential path)
+  static constexpr int kGroups = 2;
 
   // Element offset, within a unit, of the 8 elements lane `lane` loads as
   // piece p (p = 0: k slots 0-15 of row n, p = 1: slots 16-31).
   __device__ static int piece(int p, int lane) { return 32 * (lane & 31) + 16 * p + 8 * (lane >> 5); }
 
-  __device__ static void split(const uint16_t (&p)[8], h8v& hi, h8v& lo) {
+  __device__ static void split(const Packed<uint16_t, 8>& p, h8v& hi, h8v& lo) {
     const h2v k64 = {(_Float16)64.0f, (_Float16)64.0f};
     const h2v kInv = {(_Float16)0.015625f, (_Float16)0.015625f};
 #pragma unroll
     for (int r = 0; r < 8; r += 2) {
-      const h2v x = {__builtin_bit_cast(_Float16, p[r]), __builtin_bit_cast(_Float16, p[r + 1])};
+      const h2v x = __builtin_bit_cast(h2v, p.w[r >> 1]);
       const h2v t = __builtin_elementwise_trunc(x * k64) * kInv;
       const h2v d = x - t;
       hi[r] = t[0];
@@ -78,7 +81,7 @@ struct UnitScan<CodecF16> {
 
   // v[i] = inclusive prefix, within its row, of the output the lane holds
   // in accumulator i.
-  __device__ static void scan(const uint16_t (&p0)[8], const uint16_t (&p1)[8], double (&v)[16],
+  __device__ static void scan(const Packed<uint16_t, 8>& p0, const Packed<uint16_t, 8>& p1, double (&v)[16],
                               uint16_t* __restrict__ stage, int lane) {
     const int n = lane & 31, h = lane >> 5;
     uint32_t big = 0;
@@ -104,8 +107,8 @@ struct UnitScan<CodecF16> {
       // Sequential path: stage the unit in element order, then lane l
       // walks row n and keeps the prefixes of the rows m it owns.
       u32x4 w0, w1;
-      __builtin_memcpy(&w0, p0, 16);
-      __builtin_memcpy(&w1, p1, 16);
+      __builtin_memcpy(&w0, p0.w, 16);
+      __builtin_memcpy(&w1, p1.w, 16);
       *reinterpret_cast<u32x4*>(stage + piece(0, lane)) = w0;
       *reinterpret_cast<u32x4*>(stage + piece(1, lane)) = w1;
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -125,30 +128,33 @@ struct UnitScan<CodecF16> {
   }
 };
 
-template <>
-struct UnitScan<CodecI8> {
-  static constexpr int kStageHalfs = 1;
-
-  // Lane l loads 16 bytes of row n at k = 16h .. 16h + 15 as two pieces.
-  __device__ static int piece(int p, int lane) { return 32 * (lane & 31) + 16 * (lane >> 5) + 8 * p; }
-
-  __device__ static void scan(const int8_t (&p0)[8], const int8_t (&p1)[8], int (&v)[16],
-                              uint16_t* __restrict__, int lane) {
-    const int n = lane & 31, h = lane >> 5;
-    // A bytes: slot b of this lane is k = 16h + b; L[n][k] = 1 for k <= n.
-    int ones = n - 16 * h + 1;
-    ones = ones < 0 ? 0 : (ones > 16 ? 16 : ones);
-    i32x4 a;
+// The int8 product: A bytes L[n][k] for the lane's k slots 16h .. 16h + 15.
+__device__ __forceinline__ i32x4 lower_ones_i8(int lane) {
+  const int n = lane & 31, h = lane >> 5;
+  int ones = n - 16 * h + 1;
+  ones = ones < 0 ? 0 : (ones > 16 ? 16 : ones);
+  i32x4 a;
 #pragma unroll
-    for (int w = 0; w < 4; ++w) {
-      int c = ones - 4 * w;
-      c = c < 0 ? 0 : (c > 4 ? 4 : c);
-      a[w] = (int)(0x01010101u & (uint32_t)((1ull << (8 * c)) - 1ull));
-    }
+  for (int w = 0; w < 4; ++w) {
+    int c = ones - 4 * w;
+    c = c < 0 ? 0 : (c > 4 ? 4 : c);
+    a[w] = (int)(0x01010101u & (uint32_t)((1ull << (8 * c)) - 1ull));
+  }
+  return a;
+}
+
+template <>
+struct UnitScan<CodecI8U> {
+  static constexpr int kStageHalfs = 1;
+  static constexpr int kGroups = 1;
+
+  // Lane l holds the 16 bytes of row n at k = 16h .. 16h + 15.
+  __device__ static int piece(int, int lane) { return 32 * (lane & 31) + 16 * (lane >> 5); }
+
+  __device__ static void scan(const Packed<int8_t, 16>& p, int (&v)[16], uint16_t* __restrict__, int lane) {
     i32x4 b;
-    __builtin_memcpy(&b, p0, 8);
-    __builtin_memcpy(reinterpret_cast<char*>(&b) + 8, p1, 8);
-    const i32x16 c = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b, i32x16{}, 0, 0, 0);
+    __builtin_memcpy(&b, p.w, 16);
+    const i32x16 c = __builtin_amdgcn_mfma_i32_32x32x32_i8(lower_ones_i8(lane), b, i32x16{}, 0, 0, 0);
 #pragma unroll
     for (int i = 0; i < 16; ++i) v[i] = c[i];
   }
