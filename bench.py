@@ -14,7 +14,9 @@ Printed: one JSON line with the driver's fields plus
                 whole copies of the same workload;
   extras        the other workloads measured the same way: the int8 and
                 bf16 scans, run_compress_ind on N = 2^28 fp32 values with a
-                Bernoulli(0.3) mask, and run_spmv_v2 on the 2^22 x 2^22 CSR
+                Bernoulli(0.3) mask, run_filter_greater_eq on N = 2^28 fp16
+                normals (pivot 0.5), run_searchsorted of 2^24 int32 needles
+                into 2^24 sorted values, and run_spmv_v2 on the 2^22 x 2^22 CSR
                 matrix with lognormal row lengths (mean 16, sigma 1.2,
                 clipped to [0, 20000]; about 6.7e7 nonzeros).
 `--only NAME` runs one workload for a fixed number of steps without the
@@ -122,6 +124,47 @@ def _compress_workload(n: int, S: int, steps: int, warmup: int):
     torch.cuda.empty_cache()
     return {"bytes_per_step": bytes_per_step, "wall_s": wall, "event_avg_ms": sum(ev_ms) / len(ev_ms),
             "gbs_wall": bytes_per_step * steps / wall / 1e9, "kept": k}
+
+
+def _filter_workload(n: int, S: int, steps: int, warmup: int, pivot: float = 0.5):
+    """run_filter_greater_eq on N fp16 standard normals, pivot 0.5 (about
+    31% kept): a counting pass (2N bytes read, plus one 8-byte read back that
+    sizes the output) and a fused compare-and-compact pass (2N read, 2K
+    written).  Bytes per step: 4N + 2K."""
+    import torch
+
+    import tcuscan
+
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device="cpu").manual_seed(0)
+    x = torch.randn(n, generator=g).half().to(dev)
+    k = int((x >= pivot).sum().item())
+    torch.cuda.synchronize()
+    wall, ev_ms = _timed_steps(lambda: tcuscan.run_filter_greater_eq(x, pivot, S), steps, warmup)
+    bytes_per_step = 4 * n + 2 * k
+    del x
+    torch.cuda.empty_cache()
+    return {"bytes_per_step": bytes_per_step, "wall_s": wall, "event_avg_ms": sum(ev_ms) / len(ev_ms),
+            "gbs_wall": bytes_per_step * steps / wall / 1e9, "kept": k}
+
+
+def _searchsorted_workload(len_log2: int, needles_log2: int, steps: int, warmup: int):
+    """run_searchsorted of 2^needles_log2 random int32 needles into 2^len_log2
+    sorted random int32 values; reported as needles per second."""
+    import torch
+
+    import tcuscan
+
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device="cpu").manual_seed(0)
+    a = torch.sort(torch.randint(-2**31, 2**31 - 1, (1 << len_log2,), generator=g, dtype=torch.int32)).values.to(dev)
+    v = torch.randint(-2**31, 2**31 - 1, (1 << needles_log2,), generator=g, dtype=torch.int32).to(dev)
+    torch.cuda.synchronize()
+    wall, ev_ms = _timed_steps(lambda: tcuscan.run_searchsorted(a, v), steps, warmup)
+    del a, v
+    torch.cuda.empty_cache()
+    return {"wall_s": wall, "event_avg_ms": sum(ev_ms) / len(ev_ms),
+            "gneedles_per_s": (1 << needles_log2) * steps / wall / 1e9}
 
 
 def spmv_matrix(rows: int, seed: int = 0):
@@ -367,7 +410,7 @@ def main() -> int:
     ap.add_argument("--serve-rate", type=float, default=2000.0, help="Poisson arrival rate, req/s")
     ap.add_argument("--only", choices=["scan_f16", "scan_i8", "scan_bf16", "scan_f16_mfma", "scan_i8_mfma",
                                        "scan_f16_dpp", "scan_i8_dpp", "compress_ind", "spmv", "copy_f32",
-                                       "batched", "serve"],
+                                       "batched", "serve", "filter_ge", "searchsorted"],
                     help="run one workload and print its timing only (for profilers)")
     a = ap.parse_args()
     if a.gpus != 1:
@@ -390,6 +433,10 @@ def main() -> int:
             r = _batched_workload(a.steps, a.warmup)
         elif a.only == "serve":
             r = _serve_workload(a.serve_seconds, a.serve_rate)
+        elif a.only == "filter_ge":
+            r = _filter_workload(n, a.S, a.steps, a.warmup)
+        elif a.only == "searchsorted":
+            r = _searchsorted_workload(24, 24, a.steps, a.warmup)
         else:
             r = _copy_workload(n, a.steps, a.warmup)
         print(json.dumps({"only": a.only, **{k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()}}))
@@ -413,6 +460,18 @@ def main() -> int:
             "bytes_per_step": c["bytes_per_step"], "kept": c["kept"],
             "ms_per_step": round(c["wall_s"] * 1e3 / ksteps, 4),
             "note": "includes the device-to-host count read that sizes the outputs",
+        }
+        fl = _filter_workload(n, a.S, ksteps, max(3, a.warmup // 2))
+        extras[f"filter_ge_f16_2p{a.log2n}_pivot0.5"] = {
+            "value": round(fl["gbs_wall"], 2), "unit": "GB/s",
+            "bytes_per_step": fl["bytes_per_step"], "kept": fl["kept"],
+            "ms_per_step": round(fl["wall_s"] * 1e3 / ksteps, 4),
+            "note": "count pass + fused compare-and-compact pass, with the count read back to size the output",
+        }
+        ss = _searchsorted_workload(24, 24, ksteps, max(3, a.warmup // 2))
+        extras["searchsorted_i32_2p24_into_2p24"] = {
+            "value": round(ss["gneedles_per_s"], 3), "unit": "G needles/s",
+            "ms_per_step": round(ss["wall_s"] * 1e3 / ksteps, 4),
         }
         sp = _spmv_workload(1 << 22, ksteps, max(3, a.warmup // 2))
         extras["spmv_v2_2p22_lognormal16"] = {

@@ -238,6 +238,47 @@ int tcs_compress(const void* x, int x_elem_bytes, const void* idx_in, int idx_mo
                  void* workspace, size_t workspace_bytes, void* stream);
 
 /* ------------------------------------------------------------------------
+ * Filters, where, searchsorted
+ * --------------------------------------------------------------------- */
+
+/* Comparison of x[i] with a pivot; the values are those of AscendC's CMPMODE
+ * that run_count_if takes (tests/test_count_if.py:44-50). */
+enum tcs_cmp {
+  TCS_CMP_LT = 0,
+  TCS_CMP_GT = 1,
+  TCS_CMP_EQ = 2,
+  TCS_CMP_LE = 3,
+  TCS_CMP_GE = 4,
+  TCS_CMP_NE = 5
+};
+
+/* *count_out (device int64) = number of i with (x[i] cmp pivot), compared
+ * in x's type after rounding pivot to float and then to that type; a NaN
+ * counts only for TCS_CMP_NE.  x: F16, BF16, F32.  Uses
+ * tcs_compress_workspace_bytes(n).  Replaces count_if_fp16 behind
+ * run_count_if (src/torch/torch_compare.h:33-67, src/compare.cpp:25-39) and
+ * the mask + count steps of run_filter_greater_eq / run_filter_less_eq /
+ * run_where (src/torch/torch_compress.h:319-398). */
+int tcs_count_filter(const void* x, int x_dtype, double pivot, int cmp, int64_t n, int64_t* count_out,
+                     void* workspace, size_t workspace_bytes, void* stream);
+
+/* Stable compaction of the x[i] with x[i] >= pivot (TCS_CMP_GE) or
+ * x[i] <= pivot (TCS_CMP_LE), compared as in tcs_count_filter and fused
+ * into one pass (no mask in memory): values into out_vals (x's type) and/or
+ * their positions into out_pos (int32); at least one must be given; at most
+ * out_capacity are written; n < 2^31.  Replaces run_filter_greater_eq,
+ * run_filter_less_eq (values) and run_where (positions of x >= pivot;
+ * where_fp16, src/kernels/kernel_where.h) (src/torch/torch_compress.h:319-398). */
+int tcs_filter(const void* x, int x_dtype, double pivot, int cmp, void* out_vals, int32_t* out_pos,
+               int64_t n, int64_t out_capacity, void* workspace, size_t workspace_bytes, void* stream);
+
+/* out[i] = first j with sorted[j] >= values[i] (sorted ascending, int32;
+ * sorted_len < 2^31).  Replaces searchsorted_int32 behind run_searchsorted
+ * (src/torch/torch_searchsorted.h:33-82). */
+int tcs_searchsorted(const int32_t* sorted, int64_t sorted_len, const int32_t* values, int32_t* out,
+                     int64_t num_values, void* stream);
+
+/* ------------------------------------------------------------------------
  * Batched work plans
  *
  * No counterpart in the reference, whose wrappers launch and synchronise

@@ -27,7 +27,8 @@ __all__ = [
     "scan", "scan_rows", "seg_scan_flags", "blocked_scan_cpu", "reduce_tiles",
     "work_distribution", "complete_rows", "complete_blocks", "seg_sum_indptr",
     "seg_sum_flags", "csr_gather", "spmv", "compress", "compress_ind",
-    "compress_ind_no_arange", "out_dtype_for",
+    "compress_ind_no_arange", "out_dtype_for", "compare_mask", "count_if", "filter_cmp",
+    "searchsorted",
 ]
 
 _FLOAT_IN = ("float16", "bfloat16", "float32")
@@ -371,3 +372,63 @@ def lognormal_row_lengths(rows: int, mean: float, sigma: float, cap: int, rng) -
     """Row lengths round(lognormal) with the given arithmetic mean, clipped to [0, cap]."""
     mu = math.log(mean) - sigma * sigma / 2.0
     return np.clip(np.rint(rng.lognormal(mu, sigma, size=rows)), 0, cap).astype(np.int64)
+
+
+# Comparison modes of run_count_if (AscendC CMPMODE, tests/test_count_if.py:44-50).
+CMP_LT, CMP_GT, CMP_EQ, CMP_LE, CMP_GE, CMP_NE = range(6)
+
+
+def compare_mask(x: np.ndarray, pivot: float, cmp: int) -> np.ndarray:
+    """Boolean (x cmp pivot), with the pivot rounded to float32 (the
+    reference's `float pivot` argument) and then to x's type, and the
+    comparison made in that type (src/compare.cpp:25-39 casts the pivot to
+    half; torch's `x <= pivot` on a half tensor does the same)."""
+    x = np.asarray(x).reshape(-1)
+    p = np.array(np.float32(pivot)).astype(x.dtype)
+    return [np.less, np.greater, np.equal, np.less_equal, np.greater_equal, np.not_equal][cmp](x, p)
+
+
+def count_if(x: np.ndarray, pivot: float, cmp: int) -> int:
+    """Number of i with (x[i] cmp pivot).  Reference: run_count_if
+    (src/torch/torch_compare.h:33-67, src/kernels/kernel_count_if.h)."""
+    return int(np.count_nonzero(compare_mask(x, pivot, cmp)))
+
+
+def filter_cmp(x: np.ndarray, pivot: float, cmp: int, positions: bool = False) -> np.ndarray:
+    """x[x >= pivot] (cmp CMP_GE) or x[x <= pivot] (cmp CMP_LE), compared as
+    in compare_mask; with positions=True the int32 indices of the kept
+    elements instead.
+
+    Reference: run_filter_greater_equal / run_filter_less_equal / run_where
+    (src/torch/torch_compress.h:319-398): a mask from greater_equal_fp16 or
+    torch's `x <= pivot`, then run_compress; its tests compare with
+    torch.masked_select and torch.argwhere (tests/test_filter_greater_equal.py:40-54,
+    tests/test_where.py:36-50).
+    """
+    x = np.asarray(x).reshape(-1)
+    keep = compare_mask(x, pivot, cmp)
+    if positions:
+        return np.flatnonzero(keep).astype(np.int32)
+    return x[keep]
+
+def searchsorted(sorted_arr: np.ndarray, values: np.ndarray) -> np.ndarray:
+    """First j with sorted[j] >= v for each needle v (lower bound), int32:
+    a bisection over [0, len) run for all needles at once.
+
+    Reference: run_searchsorted (src/torch/torch_searchsorted.h:21-33), whose
+    tests compare with torch.searchsorted(side='left', out_int32=True)
+    (tests/test_searchsorted.py:28-31).
+    """
+    a = np.asarray(sorted_arr).reshape(-1).astype(np.int64)
+    v = np.asarray(values).reshape(-1).astype(np.int64)
+    lo = np.zeros(v.size, dtype=np.int64)
+    hi = np.full(v.size, a.size, dtype=np.int64)
+    while True:
+        active = lo < hi
+        if not active.any():
+            break
+        mid = (lo + hi) // 2
+        right = active & (a[np.minimum(mid, max(a.size - 1, 0))] < v) if a.size else active & False
+        lo = np.where(right, mid + 1, lo)
+        hi = np.where(active & ~right, mid, hi)
+    return lo.astype(np.int32)

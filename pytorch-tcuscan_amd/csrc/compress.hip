@@ -32,12 +32,49 @@ template <> struct Bits<2> { using T = uint16_t; };
 template <> struct Bits<4> { using T = uint32_t; };
 template <> struct Bits<8> { using T = uint64_t; };
 
+// Which elements are kept: mask[i] == 1 (KeepMask), or a comparison of
+// x[i] with a pivot in x's own type (KeepCmp: the filters and `where`, with
+// the pivot rounded to that type on the host, as the reference rounds it to
+// half, src/compare.cpp greater_equal_fp16).  A NaN passes only "!=".
+struct KeepMask {
+  static constexpr bool kFromX = false;
+  const int8_t* mask;
+  template <class XT>
+  __device__ __forceinline__ bool at(const XT* __restrict__, int64_t p) const { return mask[p] == 1; }
+};
+
+template <int DT, int CMP>  // DT: TCS_F16 / TCS_BF16 / TCS_F32; CMP: a tcs_cmp value
+struct KeepCmp {
+  static constexpr bool kFromX = true;
+  float pivot;
+  template <class XT>
+  __device__ __forceinline__ bool of(XT bits) const {
+    float v;
+    if constexpr (DT == TCS_F16) {
+      v = (float)__builtin_bit_cast(_Float16, (uint16_t)bits);
+    } else if constexpr (DT == TCS_BF16) {
+      v = __uint_as_float((uint32_t)bits << 16);
+    } else {
+      v = __uint_as_float((uint32_t)bits);
+    }
+    if constexpr (CMP == TCS_CMP_LT) return v < pivot;
+    if constexpr (CMP == TCS_CMP_GT) return v > pivot;
+    if constexpr (CMP == TCS_CMP_EQ) return v == pivot;
+    if constexpr (CMP == TCS_CMP_LE) return v <= pivot;
+    if constexpr (CMP == TCS_CMP_GE) return v >= pivot;
+    return v != pivot;
+  }
+  template <class XT>
+  __device__ __forceinline__ bool at(const XT* __restrict__ x, int64_t p) const { return of(x[p]); }
+};
+
 // IDX: 0 = values only, 1 = gather idx_in, 2 = positions (int32),
-// 3 = positions + 1 (int32; the row ends of flag-delimited segments).
-template <int XB, int IB, int IDX>
+// 3 = positions + 1 (int32; the row ends of flag-delimited segments),
+// 4 = positions only (int32, no values: `where`).
+template <int XB, int IB, int IDX, class Keep>
 __global__ __launch_bounds__(kCmpNW * kWave) void compress_kernel(
     const typename Bits<XB>::T* __restrict__ x, const typename Bits<IB>::T* __restrict__ idx_in,
-    const int8_t* __restrict__ mask, typename Bits<XB>::T* __restrict__ out_x,
+    Keep keep, typename Bits<XB>::T* __restrict__ out_x,
     typename Bits<IB>::T* __restrict__ out_idx, int64_t n, int64_t cap, void* ws, uint32_t num_tiles) {
   using XT = typename Bits<XB>::T;
   using IT = typename Bits<IB>::T;
@@ -64,13 +101,17 @@ __global__ __launch_bounds__(kCmpNW * kWave) void compress_kernel(
   for (int j = 0; j < J; ++j) {
     const int64_t gb = wbase + j * GSPAN;
     if (full) {
-      load_items<int8_t, G, true>(mask, gb, n, m[j]);
+      if constexpr (!Keep::kFromX) load_items<int8_t, G, true>(keep.mask, gb, n, m[j]);
       load_items<XT, G, true>(x, gb, n, xv[j]);
       if constexpr (IDX == 1) load_items<IT, G, true>(idx_in, gb, n, iv[j]);
     } else {
-      load_items<int8_t, G, false>(mask, gb, n, m[j]);
+      if constexpr (!Keep::kFromX) load_items<int8_t, G, false>(keep.mask, gb, n, m[j]);
       load_items<XT, G, false>(x, gb, n, xv[j]);
       if constexpr (IDX == 1) load_items<IT, G, false>(idx_in, gb, n, iv[j]);
+    }
+    if constexpr (Keep::kFromX) {
+#pragma unroll
+      for (int i = 0; i < G; ++i) m[j][i] = (full || gb + i < n) && keep.of(xv[j][i]) ? 1 : 0;
     }
   }
 
@@ -98,7 +139,7 @@ __global__ __launch_bounds__(kCmpNW * kWave) void compress_kernel(
       const int64_t u0 = u * kCmpTile;
       const int64_t u1 = u0 + kCmpTile < n ? u0 + kCmpTile : n;
       int c = 0;
-      for (int64_t p = u0 + lane; p < u1; p += kWave) c += mask[p] == 1;
+      for (int64_t p = u0 + lane; p < u1; p += kWave) c += keep.at(x, p) ? 1 : 0;
       return FlagVal<int>{u == 0 ? 1 : 0, wave_sum(c)};
     };
     uint64_t* desc = ws_desc(ws);
@@ -131,14 +172,16 @@ __global__ __launch_bounds__(kCmpNW * kWave) void compress_kernel(
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   };
+  if constexpr (IDX != 4) {
 #pragma unroll
-  for (int j = 0; j < J; ++j) {
-    int pos = gex[j];
+    for (int j = 0; j < J; ++j) {
+      int pos = gex[j];
 #pragma unroll
-    for (int i = 0; i < G; ++i)
-      if (m[j][i] == 1) stage[pos++] = (ST)xv[j][i];
+      for (int i = 0; i < G; ++i)
+        if (m[j][i] == 1) stage[pos++] = (ST)xv[j][i];
+    }
+    flush(out_x);
   }
-  flush(out_x);
   if constexpr (IDX != 0) {
 #pragma unroll
     for (int j = 0; j < J; ++j) {
@@ -148,7 +191,7 @@ __global__ __launch_bounds__(kCmpNW * kWave) void compress_kernel(
       for (int i = 0; i < G; ++i) {
         if (m[j][i] == 1) {
           if constexpr (IDX == 1) stage[pos] = (ST)iv[j][i];
-          if constexpr (IDX == 2) stage[pos] = (ST)(IT)(gb + i);
+          if constexpr (IDX == 2 || IDX == 4) stage[pos] = (ST)(IT)(gb + i);
           if constexpr (IDX == 3) stage[pos] = (ST)(IT)(gb + i + 1);
           ++pos;
         }
@@ -163,22 +206,35 @@ __global__ __launch_bounds__(kCmpNW * kWave) void compress_kernel(
 constexpr int kCountBlock = 256;
 constexpr int64_t kCountChunk = 1 << 16;
 
-__global__ __launch_bounds__(kCountBlock) void count_partials_kernel(const int8_t* __restrict__ mask,
+template <class XT, class Keep>
+__global__ __launch_bounds__(kCountBlock) void count_partials_kernel(const XT* __restrict__ x, Keep keep,
                                                                      int64_t n,
                                                                      int64_t* __restrict__ partial) {
   const int64_t c0 = (int64_t)blockIdx.x * kCountChunk;
   const int64_t c1 = c0 + kCountChunk < n ? c0 + kCountChunk : n;
   int c = 0;
   const bool full = c1 - c0 == kCountChunk;
-  for (int64_t b = c0 + (int64_t)threadIdx.x * 16; b < c1; b += kCountBlock * 16) {
-    int8_t m[16];
-    if (full) {
-      load_items<int8_t, 16, true>(mask, b, c1, m);
-    } else {
-      load_items<int8_t, 16, false>(mask, b, c1, m);
-    }
+  constexpr int V = Keep::kFromX ? (int)(16 / sizeof(XT)) : 16;  // items per 16-byte load
+  for (int64_t b = c0 + (int64_t)threadIdx.x * V; b < c1; b += kCountBlock * V) {
+    if constexpr (Keep::kFromX) {
+      XT v[V];
+      if (full) {
+        load_items<XT, V, true>(x, b, c1, v);
+      } else {
+        load_items<XT, V, false>(x, b, c1, v);
+      }
 #pragma unroll
-    for (int i = 0; i < 16; ++i) c += m[i] == 1;
+      for (int i = 0; i < V; ++i) c += (full || b + i < c1) && keep.of(v[i]) ? 1 : 0;
+    } else {
+      int8_t m[16];
+      if (full) {
+        load_items<int8_t, 16, true>(keep.mask, b, c1, m);
+      } else {
+        load_items<int8_t, 16, false>(keep.mask, b, c1, m);
+      }
+#pragma unroll
+      for (int i = 0; i < 16; ++i) c += m[i] == 1;
+    }
   }
   __shared__ int s[kCountBlock / kWave];
   const int w = wave_sum(c);
@@ -250,28 +306,87 @@ __global__ __launch_bounds__(256) void csr_gather_kernel(const T* __restrict__ v
   }
 }
 
-template <int XB, int IB, int IDX>
-static int launch_compress(const void* x, const void* idx_in, const int8_t* mask, void* out_x,
+template <int XB, int IB, int IDX, class Keep = KeepMask>
+static int launch_compress(const void* x, const void* idx_in, Keep keep, void* out_x,
                            void* out_idx, int64_t n, int64_t cap, void* ws, uint32_t tiles,
                            hipStream_t st) {
-  hipLaunchKernelGGL((compress_kernel<XB, IB, IDX>), dim3(tiles), dim3(kCmpNW * kWave), 0, st,
+  hipLaunchKernelGGL((compress_kernel<XB, IB, IDX, Keep>), dim3(tiles), dim3(kCmpNW * kWave), 0, st,
                      static_cast<const typename Bits<XB>::T*>(x),
-                     static_cast<const typename Bits<IB>::T*>(idx_in), mask,
+                     static_cast<const typename Bits<IB>::T*>(idx_in), keep,
                      static_cast<typename Bits<XB>::T*>(out_x),
                      static_cast<typename Bits<IB>::T*>(out_idx), n, cap, ws, tiles);
   TCS_CHECK_LAUNCH();
   return TCS_OK;
 }
 
+// Count pass of either kind of compaction into *count_out.
+template <class XT, class Keep>
+static int launch_count(const XT* x, Keep keep, int64_t n, int64_t* count_out, void* ws, hipStream_t st) {
+  const int64_t np = (n + kCountChunk - 1) / kCountChunk;
+  // Partials after the reserved header.
+  auto* partial = reinterpret_cast<int64_t*>(static_cast<char*>(ws) + kHeaderBytes);
+  if (np > 0) {
+    hipLaunchKernelGGL((count_partials_kernel<XT, Keep>), dim3((uint32_t)np), dim3(kCountBlock), 0, st, x,
+                       keep, n, partial);
+    TCS_CHECK_LAUNCH();
+  }
+  hipLaunchKernelGGL(count_final_kernel, dim3(1), dim3(kCountBlock), 0, st, partial, np, count_out);
+  TCS_CHECK_LAUNCH();
+  return TCS_OK;
+}
+
+// Pivot rounded to the input type (through float first, as a float pivot
+// argument is), so the device compares in that type.
+static float pivot_in_type(double pivot, int dt) {
+  const float f = (float)pivot;
+  if (dt == TCS_F16) return (float)(_Float16)f;
+  if (dt == TCS_BF16) {
+    uint32_t b = __builtin_bit_cast(uint32_t, f);
+    if ((b & 0x7f800000u) != 0x7f800000u) b += 0x7fffu + ((b >> 16) & 1u);  // round to nearest even
+    return __builtin_bit_cast(float, b & 0xffff0000u);
+  }
+  return f;
+}
+
+template <int DT, int CMP>
+static int filter_launch(const void* x, float pv, void* out_vals, int32_t* out_pos, int64_t n, int64_t cap,
+                         void* ws, uint32_t tiles, hipStream_t st) {
+  constexpr int XB = DT == TCS_F32 ? 4 : 2;
+  const KeepCmp<DT, CMP> keep{pv};
+  if (!out_pos) return launch_compress<XB, 4, 0>(x, nullptr, keep, out_vals, nullptr, n, cap, ws, tiles, st);
+  if (!out_vals) return launch_compress<XB, 4, 4>(x, nullptr, keep, nullptr, out_pos, n, cap, ws, tiles, st);
+  return launch_compress<XB, 4, 2>(x, nullptr, keep, out_vals, out_pos, n, cap, ws, tiles, st);
+}
+
+template <int DT, int CMP>
+static int filter_count(const void* x, float pv, int64_t n, int64_t* count_out, void* ws, hipStream_t st) {
+  using XT = typename Bits<DT == TCS_F32 ? 4 : 2>::T;
+  return launch_count(static_cast<const XT*>(x), KeepCmp<DT, CMP>{pv}, n, count_out, ws, st);
+}
+
+template <int DT>
+static int filter_count_any(int cmp, const void* x, float pv, int64_t n, int64_t* count_out, void* ws,
+                            hipStream_t st) {
+  switch (cmp) {
+    case TCS_CMP_LT: return filter_count<DT, TCS_CMP_LT>(x, pv, n, count_out, ws, st);
+    case TCS_CMP_GT: return filter_count<DT, TCS_CMP_GT>(x, pv, n, count_out, ws, st);
+    case TCS_CMP_EQ: return filter_count<DT, TCS_CMP_EQ>(x, pv, n, count_out, ws, st);
+    case TCS_CMP_LE: return filter_count<DT, TCS_CMP_LE>(x, pv, n, count_out, ws, st);
+    case TCS_CMP_GE: return filter_count<DT, TCS_CMP_GE>(x, pv, n, count_out, ws, st);
+    default: return filter_count<DT, TCS_CMP_NE>(x, pv, n, count_out, ws, st);
+  }
+}
+
 template <int XB>
 static int compress_idx(int idx_mode, int idx_bytes, const void* x, const void* idx_in,
                         const int8_t* mask, void* out_x, void* out_idx, int64_t n, int64_t cap,
                         void* ws, uint32_t tiles, hipStream_t st) {
-  if (idx_mode == 0) return launch_compress<XB, 4, 0>(x, nullptr, mask, out_x, nullptr, n, cap, ws, tiles, st);
-  if (idx_mode == 2) return launch_compress<XB, 4, 2>(x, nullptr, mask, out_x, out_idx, n, cap, ws, tiles, st);
-  if (idx_mode == 3) return launch_compress<XB, 4, 3>(x, nullptr, mask, out_x, out_idx, n, cap, ws, tiles, st);
-  if (idx_bytes == 4) return launch_compress<XB, 4, 1>(x, idx_in, mask, out_x, out_idx, n, cap, ws, tiles, st);
-  if (idx_bytes == 8) return launch_compress<XB, 8, 1>(x, idx_in, mask, out_x, out_idx, n, cap, ws, tiles, st);
+  const KeepMask keep{mask};
+  if (idx_mode == 0) return launch_compress<XB, 4, 0>(x, nullptr, keep, out_x, nullptr, n, cap, ws, tiles, st);
+  if (idx_mode == 2) return launch_compress<XB, 4, 2>(x, nullptr, keep, out_x, out_idx, n, cap, ws, tiles, st);
+  if (idx_mode == 3) return launch_compress<XB, 4, 3>(x, nullptr, keep, out_x, out_idx, n, cap, ws, tiles, st);
+  if (idx_bytes == 4) return launch_compress<XB, 4, 1>(x, idx_in, keep, out_x, out_idx, n, cap, ws, tiles, st);
+  if (idx_bytes == 8) return launch_compress<XB, 8, 1>(x, idx_in, keep, out_x, out_idx, n, cap, ws, tiles, st);
   return TCS_ERR_UNSUPPORTED_DTYPE;
 }
 
@@ -294,17 +409,51 @@ int tcs_count_mask(const int8_t* mask, int64_t n, int64_t* count_out, void* work
   if (n > 0 && !aligned16(mask)) return TCS_ERR_INVALID_ARG;
   const int64_t np = (n + kCountChunk - 1) / kCountChunk;
   if (!workspace || workspace_bytes < kHeaderBytes + (size_t)np * 8) return TCS_ERR_WORKSPACE;
+  return launch_count(mask, KeepMask{mask}, n, count_out, workspace, static_cast<hipStream_t>(stream));
+}
+
+int tcs_count_filter(const void* x, int x_dtype, double pivot, int cmp, int64_t n, int64_t* count_out,
+                     void* workspace, size_t workspace_bytes, void* stream) {
+  using namespace tcs;
+  if (n < 0 || !count_out || cmp < TCS_CMP_LT || cmp > TCS_CMP_NE || (n > 0 && !x)) return TCS_ERR_INVALID_ARG;
+  if (n > 0 && !aligned16(x)) return TCS_ERR_INVALID_ARG;
+  if (x_dtype != TCS_F16 && x_dtype != TCS_BF16 && x_dtype != TCS_F32) return TCS_ERR_UNSUPPORTED_DTYPE;
+  const int64_t np = (n + kCountChunk - 1) / kCountChunk;
+  if (!workspace || workspace_bytes < kHeaderBytes + (size_t)np * 8) return TCS_ERR_WORKSPACE;
   hipStream_t st = static_cast<hipStream_t>(stream);
-  // Partials after the reserved header (the first bytes of a workspace hold
-  // the persistent scan's ticket word).
-  auto* partial = reinterpret_cast<int64_t*>(static_cast<char*>(workspace) + kHeaderBytes);
-  if (np > 0) {
-    hipLaunchKernelGGL(count_partials_kernel, dim3((uint32_t)np), dim3(kCountBlock), 0, st, mask, n, partial);
-    TCS_CHECK_LAUNCH();
+  const float pv = pivot_in_type(pivot, x_dtype);
+  switch (x_dtype) {
+    case TCS_F16: return filter_count_any<TCS_F16>(cmp, x, pv, n, count_out, workspace, st);
+    case TCS_BF16: return filter_count_any<TCS_BF16>(cmp, x, pv, n, count_out, workspace, st);
+    default: return filter_count_any<TCS_F32>(cmp, x, pv, n, count_out, workspace, st);
   }
-  hipLaunchKernelGGL(count_final_kernel, dim3(1), dim3(kCountBlock), 0, st, partial, np, count_out);
-  TCS_CHECK_LAUNCH();
-  return TCS_OK;
+}
+
+int tcs_filter(const void* x, int x_dtype, double pivot, int cmp, void* out_vals, int32_t* out_pos,
+               int64_t n, int64_t out_capacity, void* workspace, size_t workspace_bytes, void* stream) {
+  using namespace tcs;
+  if (n < 0 || out_capacity < 0 || (cmp != TCS_CMP_GE && cmp != TCS_CMP_LE)) return TCS_ERR_INVALID_ARG;
+  if (x_dtype != TCS_F16 && x_dtype != TCS_BF16 && x_dtype != TCS_F32) return TCS_ERR_UNSUPPORTED_DTYPE;
+  if (n == 0) return TCS_OK;
+  if (!x || (!out_vals && !out_pos) || !aligned16(x)) return TCS_ERR_INVALID_ARG;
+  if (n >= ((int64_t)1 << 31)) return TCS_ERR_INVALID_ARG;
+  const uint64_t tiles = (uint64_t)((n + kCmpTile - 1) / kCmpTile);
+  if (!workspace || workspace_bytes < lookback_ws_bytes(tiles)) return TCS_ERR_WORKSPACE;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const uint32_t nt = (uint32_t)tiles;
+  const float pv = pivot_in_type(pivot, x_dtype);
+  const bool ge = cmp == TCS_CMP_GE;
+  switch (x_dtype) {
+    case TCS_F16:
+      return ge ? filter_launch<TCS_F16, TCS_CMP_GE>(x, pv, out_vals, out_pos, n, out_capacity, workspace, nt, st)
+                : filter_launch<TCS_F16, TCS_CMP_LE>(x, pv, out_vals, out_pos, n, out_capacity, workspace, nt, st);
+    case TCS_BF16:
+      return ge ? filter_launch<TCS_BF16, TCS_CMP_GE>(x, pv, out_vals, out_pos, n, out_capacity, workspace, nt, st)
+                : filter_launch<TCS_BF16, TCS_CMP_LE>(x, pv, out_vals, out_pos, n, out_capacity, workspace, nt, st);
+    default:
+      return ge ? filter_launch<TCS_F32, TCS_CMP_GE>(x, pv, out_vals, out_pos, n, out_capacity, workspace, nt, st)
+                : filter_launch<TCS_F32, TCS_CMP_LE>(x, pv, out_vals, out_pos, n, out_capacity, workspace, nt, st);
+  }
 }
 
 int tcs_compress(const void* x, int x_elem_bytes, const void* idx_in, int idx_mode, int idx_elem_bytes,

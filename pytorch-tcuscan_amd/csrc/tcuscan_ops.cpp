@@ -820,6 +820,94 @@ class BatchPlan {
   int device_ = -1;
 };
 
+// ---------------------------------------------------------------------------
+// Filters, where, searchsorted (reference: src/torch/torch_compress.h:319-398,
+// src/torch/torch_searchsorted.h:33-82)
+// ---------------------------------------------------------------------------
+
+// x[x >= pivot] / x[x <= pivot] (values) or the int32 positions of the kept
+// elements; pivot compared in x's type.  One count pass sizes the output
+// (the reference reads a count back the same way), then one fused
+// compare-and-compact pass.
+at::Tensor filter_impl(const at::Tensor& x_in, double pivot, int cmp, bool values, int S, const char* op) {
+  require_device(x_in, op, "x");
+  TORCH_CHECK(S > 0, op, ": S must be positive");
+  const auto t = x_in.scalar_type();
+  TORCH_CHECK(t == at::kHalf || t == at::kBFloat16 || t == at::kFloat, op,
+              ": x must be float16, bfloat16 or float32, got ", t);
+  at::Tensor x = prep(x_in);
+  const int64_t n = x.numel();
+  TORCH_CHECK(n < ((int64_t)1 << 31), op, ": at most 2^31 - 1 elements");
+  auto ws = workspace(x, tcs_compress_workspace_bytes(n));
+  const at::hip::HIPGuardMasqueradingAsCUDA guard(x.device());
+  at::Tensor cnt = at::zeros({1}, x.options().dtype(at::kLong));
+  if (n > 0)
+    check(tcs_count_filter(x.data_ptr(), dtype_code(t), pivot, cmp, n, cnt.data_ptr<int64_t>(), ws.first,
+                           ws.second, stream_of(x)),
+          op);
+  const int64_t k = cnt.item<int64_t>();
+  at::Tensor out = values ? at::empty({k}, x.options()) : at::empty({k}, x.options().dtype(at::kInt));
+  if (k > 0)
+    check(tcs_filter(x.data_ptr(), dtype_code(t), pivot, cmp, values ? out.data_ptr() : nullptr,
+                     values ? nullptr : out.data_ptr<int32_t>(), n, k, ws.first, ws.second, stream_of(x)),
+          op);
+  return out;
+}
+
+// src/torch/torch_compress.h:319-322
+at::Tensor run_filter_greater_equal(const at::Tensor& x, float pivot, int S) {
+  return filter_impl(x, pivot, TCS_CMP_GE, true, S, "run_filter_greater_eq");
+}
+
+// src/torch/torch_compress.h:332-335
+at::Tensor run_filter_less_equal(const at::Tensor& x, float pivot, int S) {
+  return filter_impl(x, pivot, TCS_CMP_LE, true, S, "run_filter_less_eq");
+}
+
+// src/torch/torch_compress.h:345-398: int32 indices where x >= pivot.
+at::Tensor run_where(const at::Tensor& x, float pivot, int S) {
+  return filter_impl(x, pivot, TCS_CMP_GE, false, S, "run_where");
+}
+
+// src/torch/torch_compare.h:33-67: int32 [1] count of (x cmp pivot); the
+// reference's tile_len is a tiling hint with no counterpart here.
+at::Tensor run_count_if(const at::Tensor& x_in, float pivot, uint32_t tile_len, uint8_t compare_mode) {
+  const char* op = "run_count_if";
+  require_device(x_in, op, "x");
+  TORCH_CHECK(tile_len > 0, op, ": tile_len must be positive");
+  TORCH_CHECK(compare_mode <= TCS_CMP_NE, op, ": compare_mode must be 0..5 (LT, GT, EQ, LE, GE, NE)");
+  const auto t = x_in.scalar_type();
+  TORCH_CHECK(t == at::kHalf || t == at::kBFloat16 || t == at::kFloat, op,
+              ": x must be float16, bfloat16 or float32, got ", t);
+  at::Tensor x = prep(x_in);
+  const int64_t n = x.numel();
+  auto ws = workspace(x, tcs_compress_workspace_bytes(n));
+  const at::hip::HIPGuardMasqueradingAsCUDA guard(x.device());
+  at::Tensor cnt = at::zeros({1}, x.options().dtype(at::kLong));
+  if (n > 0)
+    check(tcs_count_filter(x.data_ptr(), dtype_code(t), pivot, compare_mode, n, cnt.data_ptr<int64_t>(),
+                           ws.first, ws.second, stream_of(x)),
+          op);
+  return cnt.to(at::kInt);
+}
+
+// src/torch/torch_searchsorted.h:33-82
+at::Tensor run_searchsorted(const at::Tensor& sorted_in, const at::Tensor& values_in) {
+  const char* op = "run_searchsorted";
+  require_device(sorted_in, op, "sorted");
+  require_device(values_in, op, "values");
+  TORCH_CHECK(sorted_in.scalar_type() == at::kInt, op, ": sorted must be int32, got ", sorted_in.scalar_type());
+  TORCH_CHECK(values_in.scalar_type() == at::kInt, op, ": values must be int32, got ", values_in.scalar_type());
+  TORCH_CHECK(sorted_in.numel() < ((int64_t)1 << 31), op, ": at most 2^31 - 1 sorted elements");
+  at::Tensor sorted = sorted_in.contiguous(), values = values_in.contiguous();
+  at::Tensor out = at::empty({values.numel()}, values.options());  // flat, as the reference returns
+  const at::hip::HIPGuardMasqueradingAsCUDA guard(values.device());
+  check(tcs_searchsorted(sorted.data_ptr<int32_t>(), sorted.numel(), values.data_ptr<int32_t>(),
+                         out.data_ptr<int32_t>(), values.numel(), stream_of(values)),
+        op);
+  return out;
+}
+
 void register_server(py::module& m);  // serve.cpp
 
 }  // namespace tcuscan
@@ -852,6 +940,12 @@ PYBIND11_MODULE(tcuscan_ops, m) {
   m.def("run_row_scan", &tcuscan::run_row_scan, "Row scan of each S-row");
   m.def("run_block_scan", &tcuscan::run_block_scan, "Block scan on blocks of length S^2");
   m.def("run_scan_multi_cube", &tcuscan::run_scan_multi_cube, "Multi-cube scan");
+  m.def("run_filter_greater_eq", &tcuscan::run_filter_greater_equal, "Filter by great-equal pivot.");
+  m.def("run_filter_less_eq", &tcuscan::run_filter_less_equal, "Filter by less-equal pivot.");
+  m.def("run_where", &tcuscan::run_where, "Indices where x >= pivot (int32).");
+  m.def("run_count_if", &tcuscan::run_count_if, "Count if");
+  m.def("run_searchsorted", &tcuscan::run_searchsorted, py::arg("sorted"), py::arg("values"),
+        "Binary search (lower_bound) into a sorted int32 array");
   m.def("run_scan_ex", &tcuscan::run_scan_ex, py::arg("x"), py::arg("S"), py::arg("tiles_per_wg") = 0,
         py::arg("method") = "auto", py::arg("starting_sum") = 0.0, py::arg("lookback_window") = 0,
         "Scan with an explicit tile size, local scan method ('auto', 'dpp', 'mfma') and look-back window");
