@@ -23,8 +23,10 @@
 
 namespace tcs {
 
-constexpr int kCmpNW = 8, kCmpJ = 2, kCmpG = 8;
-constexpr int64_t kCmpTile = (int64_t)kCmpNW * kCmpJ * kWave * kCmpG;  // 8192
+constexpr int kCmpNW = 8, kCmpG = 8;
+// The smallest tile of any compaction (2 groups per wave); workspaces are
+// sized for this many tiles.
+constexpr int64_t kCmpTile = (int64_t)kCmpNW * 2 * kWave * kCmpG;  // 8192
 
 template <int BYTES> struct Bits;
 template <> struct Bits<1> { using T = uint8_t; };
@@ -71,6 +73,21 @@ struct KeepCmp {
 // IDX: 0 = values only, 1 = gather idx_in, 2 = positions (int32),
 // 3 = positions + 1 (int32; the row ends of flag-delimited segments),
 // 4 = positions only (int32, no values: `where`).
+//
+// Tile geometry: NW waves x J groups x 64 lanes x G items.  J is chosen so
+// a tile reads about 64 KiB (fewer, larger tiles amortise the per-tile
+// look-back) while one wave's LDS staging stays within 8 KiB.
+template <int XB, int IB, int IDX, bool FROMX>
+struct CmpGeom {
+  static constexpr int kInBytes = XB + (FROMX ? 0 : 1) + (IDX == 1 ? IB : 0);
+  static constexpr int kStageBytes = IDX == 0 ? XB : (IDX == 4 ? IB : (XB > IB ? XB : IB));
+  static constexpr int kByInput = kInBytes <= 2 ? 8 : (kInBytes <= 5 ? 4 : 2);
+  static constexpr int kByLds = 16 / kStageBytes;
+  static constexpr int J = kByInput < kByLds ? kByInput : kByLds;
+  static constexpr int64_t kTile = (int64_t)kCmpNW * J * kWave * kCmpG;
+  static_assert(kTile >= kCmpTile, "workspaces are sized for 8192-element tiles");
+};
+
 template <int XB, int IB, int IDX, class Keep>
 __global__ __launch_bounds__(kCmpNW * kWave) void compress_kernel(
     const typename Bits<XB>::T* __restrict__ x, const typename Bits<IB>::T* __restrict__ idx_in,
@@ -78,9 +95,11 @@ __global__ __launch_bounds__(kCmpNW * kWave) void compress_kernel(
     typename Bits<IB>::T* __restrict__ out_idx, int64_t n, int64_t cap, void* ws, uint32_t num_tiles) {
   using XT = typename Bits<XB>::T;
   using IT = typename Bits<IB>::T;
-  constexpr int G = kCmpG, J = kCmpJ, NW = kCmpNW;
+  using Geom = CmpGeom<XB, IB, IDX, Keep::kFromX>;
+  constexpr int G = kCmpG, J = Geom::J, NW = kCmpNW;
+  constexpr int64_t TILE = Geom::kTile;
   constexpr int64_t GSPAN = (int64_t)kWave * G, WSEG = GSPAN * J;
-  using ST = typename Bits<(XB > IB ? XB : IB)>::T;
+  using ST = typename Bits<Geom::kStageBytes>::T;
   __shared__ int s_wval[NW], s_wexcl[NW];
   __shared__ int64_t s_base;
   __shared__ uint32_t s_tag;
@@ -90,29 +109,38 @@ __global__ __launch_bounds__(kCmpNW * kWave) void compress_kernel(
   const int lane = __lane_id();
   const int wid = threadIdx.x / kWave;
   const uint32_t tile = blockIdx.x;
-  const int64_t t0 = (int64_t)tile * kCmpTile;
+  const int64_t t0 = (int64_t)tile * TILE;
   const int64_t wbase = t0 + (int64_t)wid * WSEG + (int64_t)lane * G;
-  const bool full = t0 + kCmpTile <= n;
+  const bool full = t0 + TILE <= n;
 
-  int8_t m[J][G];
-  XT xv[J][G];
-  IT iv[J][G];
+  uint32_t mb[J];  // bit i: item i of group j is kept
+  Packed<XT, G> xv[J];
+  IT iv[IDX == 1 ? J : 1][G];
 #pragma unroll
   for (int j = 0; j < J; ++j) {
     const int64_t gb = wbase + j * GSPAN;
+    Packed<int8_t, G> m;
     if (full) {
-      if constexpr (!Keep::kFromX) load_items<int8_t, G, true>(keep.mask, gb, n, m[j]);
+      if constexpr (!Keep::kFromX) load_items<int8_t, G, true>(keep.mask, gb, n, m);
       load_items<XT, G, true>(x, gb, n, xv[j]);
       if constexpr (IDX == 1) load_items<IT, G, true>(idx_in, gb, n, iv[j]);
     } else {
-      if constexpr (!Keep::kFromX) load_items<int8_t, G, false>(keep.mask, gb, n, m[j]);
+      if constexpr (!Keep::kFromX) load_items<int8_t, G, false>(keep.mask, gb, n, m);
       load_items<XT, G, false>(x, gb, n, xv[j]);
       if constexpr (IDX == 1) load_items<IT, G, false>(idx_in, gb, n, iv[j]);
     }
-    if constexpr (Keep::kFromX) {
+    uint32_t b = 0;
 #pragma unroll
-      for (int i = 0; i < G; ++i) m[j][i] = (full || gb + i < n) && keep.of(xv[j][i]) ? 1 : 0;
+    for (int i = 0; i < G; ++i) {
+      bool k;
+      if constexpr (Keep::kFromX) {
+        k = (full || gb + i < n) && keep.of(xv[j][i]);
+      } else {
+        k = m[i] == 1;  // items past n load as 0
+      }
+      b |= (k ? 1u : 0u) << i;
     }
+    mb[j] = b;
   }
 
   // Kept counts: per group, the lane's exclusive offset inside the wave.
@@ -120,9 +148,7 @@ __global__ __launch_bounds__(kCmpNW * kWave) void compress_kernel(
   int wtot = 0;
 #pragma unroll
   for (int j = 0; j < J; ++j) {
-    int c = 0;
-#pragma unroll
-    for (int i = 0; i < G; ++i) c += m[j][i] == 1;
+    const int c = __builtin_popcount(mb[j]);
     const int incl = wave_incl_scan(c);
     gex[j] = wtot + incl - c;
     wtot += readlane(incl, kWave - 1);
@@ -136,8 +162,8 @@ __global__ __launch_bounds__(kCmpNW * kWave) void compress_kernel(
     const int agg = readlane(incl, NW - 1);
     if (lane < NW) s_wexcl[lane] = incl - a;
     auto fb = [&](int64_t u) {
-      const int64_t u0 = u * kCmpTile;
-      const int64_t u1 = u0 + kCmpTile < n ? u0 + kCmpTile : n;
+      const int64_t u0 = u * TILE;
+      const int64_t u1 = u0 + TILE < n ? u0 + TILE : n;
       int c = 0;
       for (int64_t p = u0 + lane; p < u1; p += kWave) c += keep.at(x, p) ? 1 : 0;
       return FlagVal<int>{u == 0 ? 1 : 0, wave_sum(c)};
@@ -178,7 +204,7 @@ __global__ __launch_bounds__(kCmpNW * kWave) void compress_kernel(
       int pos = gex[j];
 #pragma unroll
       for (int i = 0; i < G; ++i)
-        if (m[j][i] == 1) stage[pos++] = (ST)xv[j][i];
+        if ((mb[j] >> i) & 1u) stage[pos++] = (ST)xv[j][i];
     }
     flush(out_x);
   }
@@ -189,7 +215,7 @@ __global__ __launch_bounds__(kCmpNW * kWave) void compress_kernel(
       const int64_t gb = wbase + j * GSPAN;
 #pragma unroll
       for (int i = 0; i < G; ++i) {
-        if (m[j][i] == 1) {
+        if ((mb[j] >> i) & 1u) {
           if constexpr (IDX == 1) stage[pos] = (ST)iv[j][i];
           if constexpr (IDX == 2 || IDX == 4) stage[pos] = (ST)(IT)(gb + i);
           if constexpr (IDX == 3) stage[pos] = (ST)(IT)(gb + i + 1);
@@ -308,8 +334,12 @@ __global__ __launch_bounds__(256) void csr_gather_kernel(const T* __restrict__ v
 
 template <int XB, int IB, int IDX, class Keep = KeepMask>
 static int launch_compress(const void* x, const void* idx_in, Keep keep, void* out_x,
-                           void* out_idx, int64_t n, int64_t cap, void* ws, uint32_t tiles,
+                           void* out_idx, int64_t n, int64_t cap, void* ws, uint32_t,
                            hipStream_t st) {
+  // The workspace holds descriptors for n / kCmpTile tiles, at least as many
+  // as this instantiation's larger tiles need.
+  constexpr int64_t kTile = CmpGeom<XB, IB, IDX, Keep::kFromX>::kTile;
+  const uint32_t tiles = (uint32_t)((n + kTile - 1) / kTile);
   hipLaunchKernelGGL((compress_kernel<XB, IB, IDX, Keep>), dim3(tiles), dim3(kCmpNW * kWave), 0, st,
                      static_cast<const typename Bits<XB>::T*>(x),
                      static_cast<const typename Bits<IB>::T*>(idx_in), keep,

@@ -130,12 +130,14 @@ __device__ __forceinline__ void store_items(T* __restrict__ p, int64_t base, int
 // where they are used.
 template <class T, int N>
 struct Packed {
-  static_assert(sizeof(T) == 1 || sizeof(T) == 2 || sizeof(T) == 4, "element size");
+  static_assert(sizeof(T) == 1 || sizeof(T) == 2 || sizeof(T) == 4 || sizeof(T) == 8, "element size");
   static_assert((N * sizeof(T)) % 4 == 0, "whole words");
   static constexpr int kW = (int)(N * sizeof(T) / 4);
   uint32_t w[kW];
   __device__ __forceinline__ T operator[](int i) const {
-    if constexpr (sizeof(T) == 4) {
+    if constexpr (sizeof(T) == 8) {
+      return __builtin_bit_cast(T, (uint64_t)w[2 * i] | ((uint64_t)w[2 * i + 1] << 32));
+    } else if constexpr (sizeof(T) == 4) {
       return __builtin_bit_cast(T, w[i]);
     } else if constexpr (sizeof(T) == 2) {
       return __builtin_bit_cast(T, (uint16_t)(w[i >> 1] >> (16 * (i & 1))));
