@@ -272,6 +272,21 @@ int tcs_count_filter(const void* x, int x_dtype, double pivot, int cmp, int64_t 
 int tcs_filter(const void* x, int x_dtype, double pivot, int cmp, void* out_vals, int32_t* out_pos,
                int64_t n, int64_t out_capacity, void* workspace, size_t workspace_bytes, void* stream);
 
+/* ------------------------------------------------------------------------
+ * Split (stable partition)
+ * --------------------------------------------------------------------- */
+
+size_t tcs_split_workspace_bytes(int64_t n);
+
+/* out_x = [x[i] for mask[i] == 1] ++ [x[i] for mask[i] != 1], both parts in
+ * input order; with idx_in (int32, may be NULL) out_idx receives idx_in
+ * permuted the same way.  x: elements of 1, 2, 4 or 8 bytes; mask: int8;
+ * n < 2^31; all 16-byte aligned.  Stream-ordered, no host sync.  Replaces
+ * split_uint16 / split_ind_uint16 behind run_split / run_split_ind
+ * (src/torch/torch_split.h:25-141). */
+int tcs_split(const void* x, int x_elem_bytes, const int8_t* mask, const int32_t* idx_in, void* out_x,
+              int32_t* out_idx, int64_t n, void* workspace, size_t workspace_bytes, void* stream);
+
 /* out[i] = first j with sorted[j] >= values[i] (sorted ascending, int32;
  * sorted_len < 2^31).  Replaces searchsorted_int32 behind run_searchsorted
  * (src/torch/torch_searchsorted.h:33-82). */

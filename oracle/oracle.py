@@ -28,7 +28,7 @@ __all__ = [
     "work_distribution", "complete_rows", "complete_blocks", "seg_sum_indptr",
     "seg_sum_flags", "csr_gather", "spmv", "compress", "compress_ind",
     "compress_ind_no_arange", "out_dtype_for", "compare_mask", "count_if", "filter_cmp",
-    "searchsorted",
+    "searchsorted", "split",
 ]
 
 _FLOAT_IN = ("float16", "bfloat16", "float32")
@@ -432,3 +432,19 @@ def searchsorted(sorted_arr: np.ndarray, values: np.ndarray) -> np.ndarray:
         lo = np.where(right, mid + 1, lo)
         hi = np.where(active & ~right, mid, hi)
     return lo.astype(np.int32)
+
+
+def split(x: np.ndarray, mask: np.ndarray, idx: np.ndarray | None = None):
+    """Stable partition: the x[i] with mask[i] == 1, in order, then every
+    other x[i], in order; with idx, the same permutation applied to idx.
+
+    Reference: run_split / run_split_ind (src/torch/torch_split.h:25-141);
+    tests/test_split.py:52-66 and tests/test_split_ind.py compare the parts
+    with torch.masked_select(x, mask == 1) and (x, mask == 0).
+    """
+    x = np.asarray(x).reshape(-1)
+    keep = np.asarray(mask).reshape(-1) == 1
+    order = np.concatenate([np.flatnonzero(keep), np.flatnonzero(~keep)])
+    if idx is None:
+        return x[order]
+    return x[order], np.asarray(idx).reshape(-1)[order]

@@ -869,6 +869,51 @@ at::Tensor run_where(const at::Tensor& x, float pivot, int S) {
   return filter_impl(x, pivot, TCS_CMP_GE, false, S, "run_where");
 }
 
+// ---------------------------------------------------------------------------
+// Split (reference: src/torch/torch_split.h:25-141)
+// ---------------------------------------------------------------------------
+
+std::pair<at::Tensor, at::Tensor> split_impl(const at::Tensor& x_in, const at::Tensor& mask_in,
+                                             const at::Tensor* idx_in, int S, const char* op) {
+  require_device(x_in, op, "x");
+  require_device(mask_in, op, "mask");
+  TORCH_CHECK(S > 0, op, ": S must be positive");
+  TORCH_CHECK(mask_in.scalar_type() == at::kChar, op, ": mask must be int8, got ", mask_in.scalar_type());
+  TORCH_CHECK(mask_in.numel() == x_in.numel(), op, ": mask and x must have the same length");
+  const int eb = (int)x_in.element_size();
+  TORCH_CHECK(eb == 1 || eb == 2 || eb == 4 || eb == 8, op, ": unsupported dtype ", x_in.scalar_type());
+  at::Tensor x = prep(x_in), mask = prep(mask_in), idx;
+  if (idx_in) {
+    require_device(*idx_in, op, "indices_in");
+    TORCH_CHECK(idx_in->scalar_type() == at::kInt, op, ": indices_in must be int32, got ", idx_in->scalar_type());
+    TORCH_CHECK(idx_in->numel() == x_in.numel(), op, ": indices_in and x must have the same length");
+    idx = prep(*idx_in);
+  }
+  const int64_t n = x.numel();
+  TORCH_CHECK(n < ((int64_t)1 << 31), op, ": at most 2^31 - 1 elements");
+  const at::hip::HIPGuardMasqueradingAsCUDA guard(x.device());
+  at::Tensor z = at::empty({n}, x.options());
+  at::Tensor zi = idx_in ? at::empty({n}, x.options().dtype(at::kInt)) : at::Tensor();
+  if (n > 0) {
+    auto ws = workspace(x, tcs_split_workspace_bytes(n));
+    check(tcs_split(x.data_ptr(), eb, mask.data_ptr<int8_t>(), idx_in ? idx.data_ptr<int32_t>() : nullptr,
+                    z.data_ptr(), idx_in ? zi.data_ptr<int32_t>() : nullptr, n, ws.first, ws.second,
+                    stream_of(x)),
+          op);
+  }
+  return {z, zi};
+}
+
+at::Tensor run_split(const at::Tensor& x, const at::Tensor& mask, int S) {
+  return split_impl(x, mask, nullptr, S, "run_split").first;
+}
+
+std::tuple<at::Tensor, at::Tensor> run_split_ind(const at::Tensor& x, const at::Tensor& mask,
+                                                 const at::Tensor& indices_in, int S) {
+  auto r = split_impl(x, mask, &indices_in, S, "run_split_ind");
+  return {r.first, r.second};
+}
+
 // src/torch/torch_compare.h:33-67: int32 [1] count of (x cmp pivot); the
 // reference's tile_len is a tiling hint with no counterpart here.
 at::Tensor run_count_if(const at::Tensor& x_in, float pivot, uint32_t tile_len, uint8_t compare_mode) {
@@ -944,6 +989,8 @@ PYBIND11_MODULE(tcuscan_ops, m) {
   m.def("run_filter_less_eq", &tcuscan::run_filter_less_equal, "Filter by less-equal pivot.");
   m.def("run_where", &tcuscan::run_where, "Indices where x >= pivot (int32).");
   m.def("run_count_if", &tcuscan::run_count_if, "Count if");
+  m.def("run_split", &tcuscan::run_split, "Split (stable partition by mask == 1)");
+  m.def("run_split_ind", &tcuscan::run_split_ind, "Split with indices");
   m.def("run_searchsorted", &tcuscan::run_searchsorted, py::arg("sorted"), py::arg("values"),
         "Binary search (lower_bound) into a sorted int32 array");
   m.def("run_scan_ex", &tcuscan::run_scan_ex, py::arg("x"), py::arg("S"), py::arg("tiles_per_wg") = 0,
