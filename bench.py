@@ -167,6 +167,30 @@ def _searchsorted_workload(len_log2: int, needles_log2: int, steps: int, warmup:
             "gneedles_per_s": (1 << needles_log2) * steps / wall / 1e9}
 
 
+def _sort_workload(n: int, dtype_name: str, steps: int, warmup: int):
+    """run_radix_sort on N random keys (fp16 standard normals or int32 over
+    the full range) with int32 positions; torch.sort(stable=True) of the
+    same keys timed beside it.  Reported in G keys/s."""
+    import torch
+
+    import tcuscan
+
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device="cpu").manual_seed(0)
+    if dtype_name == "fp16":
+        x = torch.randn(n, generator=g).half().to(dev)
+    else:
+        x = torch.randint(-2**31, 2**31 - 1, (n,), generator=g, dtype=torch.int32).to(dev)
+    torch.cuda.synchronize()
+    wall, ev_ms = _timed_steps(lambda: tcuscan.run_radix_sort(x, 128), steps, warmup)
+    twall, tev = _timed_steps(lambda: torch.sort(x, stable=True), max(3, steps // 2), 2, spin_s=0.1)
+    tsteps = max(3, steps // 2)
+    del x
+    torch.cuda.empty_cache()
+    return {"wall_s": wall, "event_avg_ms": sum(ev_ms) / len(ev_ms), "gkeys_per_s": n * steps / wall / 1e9,
+            "torch_sort_ms": twall * 1e3 / tsteps, "torch_gkeys_per_s": n * tsteps / twall / 1e9}
+
+
 def spmv_matrix(rows: int, seed: int = 0):
     """The SpMV workload's matrix on the GPU: rows = cols, lognormal row
     lengths (mean 16, sigma 1.2, clipped to [0, 20000]), columns uniform and
@@ -410,7 +434,7 @@ def main() -> int:
     ap.add_argument("--serve-rate", type=float, default=2000.0, help="Poisson arrival rate, req/s")
     ap.add_argument("--only", choices=["scan_f16", "scan_i8", "scan_bf16", "scan_f16_mfma", "scan_i8_mfma",
                                        "scan_f16_dpp", "scan_i8_dpp", "compress_ind", "spmv", "copy_f32",
-                                       "batched", "serve", "filter_ge", "searchsorted"],
+                                       "batched", "serve", "filter_ge", "searchsorted", "sort_f16", "sort_i32"],
                     help="run one workload and print its timing only (for profilers)")
     a = ap.parse_args()
     if a.gpus != 1:
@@ -437,6 +461,8 @@ def main() -> int:
             r = _filter_workload(n, a.S, a.steps, a.warmup)
         elif a.only == "searchsorted":
             r = _searchsorted_workload(24, 24, a.steps, a.warmup)
+        elif a.only in ("sort_f16", "sort_i32"):
+            r = _sort_workload(n, "fp16" if a.only == "sort_f16" else "int32", a.steps, a.warmup)
         else:
             r = _copy_workload(n, a.steps, a.warmup)
         print(json.dumps({"only": a.only, **{k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()}}))

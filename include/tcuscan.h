@@ -287,6 +287,27 @@ size_t tcs_split_workspace_bytes(int64_t n);
 int tcs_split(const void* x, int x_elem_bytes, const int8_t* mask, const int32_t* idx_in, void* out_x,
               int32_t* out_idx, int64_t n, void* workspace, size_t workspace_bytes, void* stream);
 
+/* ------------------------------------------------------------------------
+ * Radix sort
+ * --------------------------------------------------------------------- */
+
+/* Workspace of tcs_sort (look-back descriptors and digit counts: zeroed
+ * once, reusable like every workspace) and its scratch (one temporary copy
+ * of keys and positions, any contents); 0 for an unsupported dtype. */
+size_t tcs_sort_workspace_bytes(int64_t n, int dtype);
+size_t tcs_sort_scratch_bytes(int64_t n, int dtype);
+
+/* Stable sort of x (dtype F16, BF16, F32, I8, I16, I32 or U32) into out_x,
+ * with out_idx[k] = the input position of out_x[k] (or idx_in[that
+ * position] when idx_in is given).  Order as torch.sort(stable=True):
+ * ascending, or descending when `descending` != 0; -0.0 and +0.0 compare
+ * equal and NaN sorts after +inf.  Values keep their exact bits.  n < 2^31;
+ * x and idx_in 16-byte aligned.  Replaces radix_sort_fp16 / radix_sort_int16
+ * behind run_radix_sort (src/torch/torch_sort.h:21-81, src/radix_sort.cpp). */
+int tcs_sort(const void* x, int dtype, const int32_t* idx_in, void* out_x, int32_t* out_idx, int64_t n,
+             int descending, void* workspace, size_t workspace_bytes, void* scratch, size_t scratch_bytes,
+             void* stream);
+
 /* out[i] = first j with sorted[j] >= values[i] (sorted ascending, int32;
  * sorted_len < 2^31).  Replaces searchsorted_int32 behind run_searchsorted
  * (src/torch/torch_searchsorted.h:33-82). */

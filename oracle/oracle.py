@@ -28,7 +28,7 @@ __all__ = [
     "work_distribution", "complete_rows", "complete_blocks", "seg_sum_indptr",
     "seg_sum_flags", "csr_gather", "spmv", "compress", "compress_ind",
     "compress_ind_no_arange", "out_dtype_for", "compare_mask", "count_if", "filter_cmp",
-    "searchsorted", "split",
+    "searchsorted", "split", "sort",
 ]
 
 _FLOAT_IN = ("float16", "bfloat16", "float32")
@@ -448,3 +448,27 @@ def split(x: np.ndarray, mask: np.ndarray, idx: np.ndarray | None = None):
     if idx is None:
         return x[order]
     return x[order], np.asarray(idx).reshape(-1)[order]
+
+
+def sort(x: np.ndarray, descending: bool = False):
+    """Stable sort with the input positions: (values, int32 positions).
+
+    Order as torch.sort(stable=True): ascending or descending by value, ties
+    in input order, -0.0 equal to +0.0, NaN after everything (before
+    everything when descending).  Reference: run_radix_sort
+    (src/torch/torch_sort.h:21-81, ascending); tests/test_radix_sort.py
+    compares with torch.sort(x, descending=False).
+    """
+    x = np.asarray(x).reshape(-1)
+    idx = np.arange(x.size)
+    if x.dtype.kind == "f":
+        nan = np.isnan(x)
+        v = np.where(nan, 0.0, x.astype(np.float64))
+        if descending:
+            order = np.lexsort((idx, -v, ~nan))
+        else:
+            order = np.lexsort((idx, v, nan))
+    else:
+        v = x.astype(np.int64)
+        order = np.lexsort((idx, -v if descending else v))
+    return x[order], order.astype(np.int32)

@@ -44,7 +44,7 @@ template <class IdxT, class OutT>
 __global__ __launch_bounds__(256) void seg_rows_kernel(const IdxT* __restrict__ ends, int64_t R,
                                                        int64_t n_ends, int64_t nnz,
                                                        OutT* __restrict__ y,
-                                                       uint32_t* __restrict__ tile_row,
+                                                       uint64_t* __restrict__ tile_row,
                                                        uint32_t num_tiles) {
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= R) return;
@@ -54,11 +54,11 @@ __global__ __launch_bounds__(256) void seg_rows_kernel(const IdxT* __restrict__ 
     y[r] = OutT(0);
   } else {
     for (int64_t b = (s + kSegTile - 1) / kSegTile; b * kSegTile < e && b < num_tiles; ++b)
-      tile_row[b] = (uint32_t)r;
+      tile_row[b] = (uint64_t)r;
   }
   if (r == R - 1) {
     const int64_t last = e > s ? e : s;
-    for (int64_t b = (last + kSegTile - 1) / kSegTile; b < num_tiles; ++b) tile_row[b] = (uint32_t)R;
+    for (int64_t b = (last + kSegTile - 1) / kSegTile; b < num_tiles; ++b) tile_row[b] = (uint64_t)R;
   }
 }
 
@@ -131,7 +131,7 @@ __device__ __forceinline__ int lower_bound_u32(const uint32_t* list, int cnt, ui
 template <class Src, class IdxT, class OutT>
 __global__ __launch_bounds__(kSegNW * kWave) void seg_reduce_kernel(
     Src src, const IdxT* __restrict__ ends, int64_t R, int64_t n_ends, int64_t nnz,
-    OutT* __restrict__ y, const uint32_t* __restrict__ tile_row, uint32_t num_tiles, void* ws) {
+    OutT* __restrict__ y, const uint64_t* __restrict__ tile_row, uint32_t num_tiles, void* ws) {
   using Acc = typename Src::Acc;
   constexpr int G = kSegG, J = kSegJ, NW = kSegNW;
   constexpr int64_t GSPAN = (int64_t)kWave * G, WSEG = GSPAN * J;
@@ -152,7 +152,7 @@ __global__ __launch_bounds__(kSegNW * kWave) void seg_reduce_kernel(
 
   // Rows touching the tile and the row ends around it: list[k] =
   // end(r_lo - 1 + k).
-  int64_t r_lo = tile_row[tile];
+  int64_t r_lo = (int64_t)tile_row[tile];
   int64_t r_hi = tile + 1 < num_tiles ? (int64_t)tile_row[tile + 1] : R - 1;
   if (r_lo > R - 1) r_lo = R - 1;
   if (r_hi > R - 1) r_hi = R - 1;
@@ -342,7 +342,7 @@ inline SegSumPlan seg_plan(int64_t nnz) {
   p.num_tiles = (uint32_t)((nnz + kSegTile - 1) / kSegTile);
   const size_t lb = lookback_ws_bytes(p.num_tiles);
   p.tile_row_off = lb;
-  p.ws_bytes = lb + (((size_t)p.num_tiles * 4 + 255) & ~(size_t)255);
+  p.ws_bytes = lb + (((size_t)p.num_tiles * 8 + 255) & ~(size_t)255);
   return p;
 }
 
@@ -353,7 +353,7 @@ static int launch_seg(const Src& src, const void* ends_v, int64_t R, int64_t n_e
   if (!ws || ws_bytes < plan.ws_bytes) return TCS_ERR_WORKSPACE;
   const auto* ends = static_cast<const IdxT*>(ends_v);
   auto* y = static_cast<OutT*>(y_v);
-  auto* tile_row = reinterpret_cast<uint32_t*>(static_cast<char*>(ws) + plan.tile_row_off);
+  auto* tile_row = reinterpret_cast<uint64_t*>(static_cast<char*>(ws) + plan.tile_row_off);
   const int64_t row_blocks = (R + 255) / 256;
   hipLaunchKernelGGL((seg_rows_kernel<IdxT, OutT>), dim3((uint32_t)row_blocks), dim3(256), 0, st,
                      ends, R, n_ends, nnz, y, tile_row, plan.num_tiles);

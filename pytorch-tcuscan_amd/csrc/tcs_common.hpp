@@ -206,7 +206,11 @@ __device__ __forceinline__ void reg_barrier(Packed<T, N>& r) {
 // All but the epoch are zero between calls (each launch resets them), and
 // no op puts scratch data in the header.  Look-back kernels keep 16 bytes
 // of descriptor per tile right after the header; other scratch follows the
-// descriptors.
+// descriptors.  One workspace serves many ops and sizes, so bytes that one
+// call uses as scratch a later call may read as descriptors: scratch is
+// written as 64-bit words whose high half is zero (counts, indices) or
+// that of a double or negative integer, none of which reads as a status
+// word carrying a current tag (tags are small: one per launch).
 // ---------------------------------------------------------------------------
 constexpr int kTicketCounters = 8;
 constexpr int kSlotEpoch = 8, kSlotGroupsArrived = 9, kSlotArrived0 = 10;

@@ -54,6 +54,8 @@ __device__ __forceinline__ double dpp_f64(double v) {
 template <int CTRL, int RM = 0xf, int BM = 0xf>
 __device__ __forceinline__ int dpp_mov(int v) { return dpp_i32<CTRL, RM, BM>(v); }
 template <int CTRL, int RM = 0xf, int BM = 0xf>
+__device__ __forceinline__ uint32_t dpp_mov(uint32_t v) { return (uint32_t)dpp_i32<CTRL, RM, BM>((int)v); }
+template <int CTRL, int RM = 0xf, int BM = 0xf>
 __device__ __forceinline__ float dpp_mov(float v) { return dpp_f32<CTRL, RM, BM>(v); }
 template <int CTRL, int RM = 0xf, int BM = 0xf>
 __device__ __forceinline__ double dpp_mov(double v) { return dpp_f64<CTRL, RM, BM>(v); }

@@ -914,6 +914,41 @@ std::tuple<at::Tensor, at::Tensor> run_split_ind(const at::Tensor& x, const at::
   return {r.first, r.second};
 }
 
+// ---------------------------------------------------------------------------
+// Radix sort (reference: src/torch/torch_sort.h:21-81)
+// ---------------------------------------------------------------------------
+
+std::tuple<at::Tensor, at::Tensor> sort_impl(const at::Tensor& x_in, bool descending, const char* op) {
+  require_device(x_in, op, "x");
+  const int dt = dtype_code(x_in.scalar_type());
+  TORCH_CHECK(dt >= 0 && tcs_sort_workspace_bytes(1, dt) > 0, op,
+              ": x must be float16, bfloat16, float32, int8, int16, int32 or uint32, got ", x_in.scalar_type());
+  at::Tensor x = prep(x_in);
+  const int64_t n = x.numel();
+  TORCH_CHECK(n < ((int64_t)1 << 31), op, ": at most 2^31 - 1 elements");
+  const at::hip::HIPGuardMasqueradingAsCUDA guard(x.device());
+  at::Tensor z = at::empty({n}, x.options());
+  at::Tensor zi = at::empty({n}, x.options().dtype(at::kInt));
+  if (n > 0) {
+    auto ws = workspace(x, tcs_sort_workspace_bytes(n, dt));
+    at::Tensor scratch = at::empty({(int64_t)tcs_sort_scratch_bytes(n, dt)}, x.options().dtype(at::kByte));
+    check(tcs_sort(x.data_ptr(), dt, nullptr, z.data_ptr(), zi.data_ptr<int32_t>(), n, descending ? 1 : 0,
+                   ws.first, ws.second, scratch.data_ptr(), (size_t)scratch.numel(), stream_of(x)),
+          op);
+  }
+  return {z, zi};
+}
+
+// Ascending, as the reference's kernels sort (radix_sort.cpp: descending = false).
+std::tuple<at::Tensor, at::Tensor> run_radix_sort(const at::Tensor& x, int S) {
+  TORCH_CHECK(S > 0, "run_radix_sort: S must be positive");
+  return sort_impl(x, false, "run_radix_sort");
+}
+
+std::tuple<at::Tensor, at::Tensor> run_radix_sort_ex(const at::Tensor& x, bool descending) {
+  return sort_impl(x, descending, "run_radix_sort_ex");
+}
+
 // src/torch/torch_compare.h:33-67: int32 [1] count of (x cmp pivot); the
 // reference's tile_len is a tiling hint with no counterpart here.
 at::Tensor run_count_if(const at::Tensor& x_in, float pivot, uint32_t tile_len, uint8_t compare_mode) {
@@ -990,6 +1025,9 @@ PYBIND11_MODULE(tcuscan_ops, m) {
   m.def("run_where", &tcuscan::run_where, "Indices where x >= pivot (int32).");
   m.def("run_count_if", &tcuscan::run_count_if, "Count if");
   m.def("run_split", &tcuscan::run_split, "Split (stable partition by mask == 1)");
+  m.def("run_radix_sort", &tcuscan::run_radix_sort, "Radix sort with indices (ascending, stable)");
+  m.def("run_radix_sort_ex", &tcuscan::run_radix_sort_ex, py::arg("x"), py::arg("descending") = false,
+        "Radix sort with indices, ascending or descending (stable)");
   m.def("run_split_ind", &tcuscan::run_split_ind, "Split with indices");
   m.def("run_searchsorted", &tcuscan::run_searchsorted, py::arg("sorted"), py::arg("values"),
         "Binary search (lower_bound) into a sorted int32 array");
