@@ -191,6 +191,25 @@ def _sort_workload(n: int, dtype_name: str, steps: int, warmup: int):
             "torch_sort_ms": twall * 1e3 / tsteps, "torch_gkeys_per_s": n * tsteps / twall / 1e9}
 
 
+def _topk_workload(n: int, k: int, steps: int, warmup: int):
+    """run_topk_fp16 of the k largest of N fp16 standard normals (values and
+    positions, sorted), with torch.topk of the same keys timed beside it."""
+    import torch
+
+    import tcuscan
+
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device="cpu").manual_seed(0)
+    x = torch.randn(n, generator=g).half().to(dev)
+    torch.cuda.synchronize()
+    wall, ev_ms = _timed_steps(lambda: tcuscan.run_topk_fp16(x, k, -10.0, 10.0, 128), steps, warmup)
+    twall, _ = _timed_steps(lambda: torch.topk(x, k), steps, warmup, spin_s=0.1)
+    del x
+    torch.cuda.empty_cache()
+    return {"wall_s": wall, "ms_per_call": wall * 1e3 / steps, "event_avg_ms": sum(ev_ms) / len(ev_ms),
+            "gbs_read_once": 2 * n * steps / wall / 1e9, "torch_topk_ms": twall * 1e3 / steps}
+
+
 def spmv_matrix(rows: int, seed: int = 0):
     """The SpMV workload's matrix on the GPU: rows = cols, lognormal row
     lengths (mean 16, sigma 1.2, clipped to [0, 20000]), columns uniform and
@@ -434,7 +453,8 @@ def main() -> int:
     ap.add_argument("--serve-rate", type=float, default=2000.0, help="Poisson arrival rate, req/s")
     ap.add_argument("--only", choices=["scan_f16", "scan_i8", "scan_bf16", "scan_f16_mfma", "scan_i8_mfma",
                                        "scan_f16_dpp", "scan_i8_dpp", "compress_ind", "spmv", "copy_f32",
-                                       "batched", "serve", "filter_ge", "searchsorted", "sort_f16", "sort_i32"],
+                                       "batched", "serve", "filter_ge", "searchsorted", "sort_f16", "sort_i32",
+                                       "topk_f16"],
                     help="run one workload and print its timing only (for profilers)")
     a = ap.parse_args()
     if a.gpus != 1:
@@ -461,6 +481,8 @@ def main() -> int:
             r = _filter_workload(n, a.S, a.steps, a.warmup)
         elif a.only == "searchsorted":
             r = _searchsorted_workload(24, 24, a.steps, a.warmup)
+        elif a.only == "topk_f16":
+            r = _topk_workload(n, 2048, a.steps, a.warmup)
         elif a.only in ("sort_f16", "sort_i32"):
             r = _sort_workload(n, "fp16" if a.only == "sort_f16" else "int32", a.steps, a.warmup)
         else:

@@ -308,6 +308,38 @@ int tcs_sort(const void* x, int dtype, const int32_t* idx_in, void* out_x, int32
              int descending, void* workspace, size_t workspace_bytes, void* scratch, size_t scratch_bytes,
              void* stream);
 
+/* ------------------------------------------------------------------------
+ * Top-k
+ * --------------------------------------------------------------------- */
+
+/* Workspace (zeroed once, reusable) and scratch (any contents) of tcs_topk;
+ * 0 for an unsupported dtype. */
+size_t tcs_topk_workspace_bytes(int64_t n, int64_t k, int dtype);
+size_t tcs_topk_scratch_bytes(int64_t n, int64_t k, int dtype);
+
+/* The k largest (largest != 0) or smallest elements of x, by radix select,
+ * into out_vals with their input positions in out_idx, ordered largest
+ * first (smallest first), ties by input position; among equal elements the
+ * lowest positions are taken.  Order as tcs_sort (NaN above +inf).  dtype
+ * F16, BF16, F32, I8, I16, I32 or U32; 0 <= k <= n < 2^31.  No host sync.
+ * Replaces topk_fp16 / topk_int16 behind run_topk_fp16 / run_topk_int16
+ * (src/torch/torch_topk.h:69-207), whose x_min / x_max bounds it does not
+ * need. */
+int tcs_topk(const void* x, int dtype, int64_t n, int64_t k, int largest, void* out_vals, int32_t* out_idx,
+             void* workspace, size_t workspace_bytes, void* scratch, size_t scratch_bytes, void* stream);
+
+/* Number of estimates tcs_topk_pivot writes: min(64, ceil(n / (32 *
+ * num_samples))). */
+int64_t tcs_topk_pivot_blocks(int64_t n, int num_samples);
+
+/* Per block, an fp16 estimate of the k-th largest value of x (fp16): over
+ * the block's first eight tiles of 32 * num_samples elements, the largest
+ * tile value of "4th largest of the 8th largests of each 32" (at least 32
+ * elements of that tile are >= it).  4 <= num_samples <= 64.  Replaces
+ * topk_pivot_fp16 behind run_topk_pivot_fp16 (src/torch/torch_topk.h:23-60,
+ * src/kernels/kernel_topk_pivot.h). */
+int tcs_topk_pivot(const void* x, int64_t n, int num_samples, void* out, int64_t out_len, void* stream);
+
 /* out[i] = first j with sorted[j] >= values[i] (sorted ascending, int32;
  * sorted_len < 2^31).  Replaces searchsorted_int32 behind run_searchsorted
  * (src/torch/torch_searchsorted.h:33-82). */

@@ -28,7 +28,7 @@ __all__ = [
     "work_distribution", "complete_rows", "complete_blocks", "seg_sum_indptr",
     "seg_sum_flags", "csr_gather", "spmv", "compress", "compress_ind",
     "compress_ind_no_arange", "out_dtype_for", "compare_mask", "count_if", "filter_cmp",
-    "searchsorted", "split", "sort",
+    "searchsorted", "split", "sort", "topk", "topk_pivot", "top_p_filter",
 ]
 
 _FLOAT_IN = ("float16", "bfloat16", "float32")
@@ -472,3 +472,64 @@ def sort(x: np.ndarray, descending: bool = False):
         v = x.astype(np.int64)
         order = np.lexsort((idx, -v if descending else v))
     return x[order], order.astype(np.int32)
+
+
+def topk(x: np.ndarray, k: int, largest: bool = True):
+    """The k largest (smallest) elements with their input positions, largest
+    (smallest) first; ties by input position, so among equal elements the
+    lowest positions are taken.  The first k entries of the stable sort.
+
+    Reference: run_topk_fp16 / run_topk_int16 (src/torch/torch_topk.h:69-207);
+    tests/test_topk.py compares the sorted values and sorted indices with
+    torch.topk's.
+    """
+    v, i = sort(x, descending=largest)
+    return v[:k], i[:k]
+
+
+def topk_pivot(x: np.ndarray, num_samples: int = 32, blocks: int | None = None) -> np.ndarray:
+    """Per block: over its first eight tiles of 32 * num_samples elements,
+    the largest of (per tile) the 4th largest of the 8th largests of each 32
+    consecutive elements (positions past the end count as -inf).
+
+    Reference: run_topk_pivot_fp16 (src/torch/torch_topk.h:23-60) and
+    KernelPivotTopkEstimator (src/kernels/kernel_topk_pivot.h, k_inner 8,
+    k_outer 4, eight tiles per block).  Blocks: min(64, tiles), each taking
+    ceil(tiles / blocks) consecutive tiles.  Non-NaN inputs.
+    """
+    x = np.asarray(x, dtype=np.float16).reshape(-1)
+    tile = 32 * num_samples
+    tiles = -(-x.size // tile)
+    if blocks is None:
+        blocks = min(64, tiles)
+    per = -(-tiles // blocks)
+    out = np.empty(blocks, dtype=np.float16)
+    for b in range(blocks):
+        best = None
+        for it in range(min(8, per)):
+            base = (b * per + it) * tile
+            if base >= x.size:
+                break
+            t = np.full(tile, -np.inf, dtype=np.float16)
+            seg = x[base:base + tile]
+            t[:seg.size] = seg
+            g8 = np.sort(t.reshape(num_samples, 32), axis=1)[:, ::-1][:, 7]
+            est = np.sort(g8)[::-1][3]
+            best = est if best is None else max(best, est)
+        out[b] = best if best is not None else np.float16(-np.inf)
+    return out
+
+
+def top_p_filter(probs: np.ndarray, p: float):
+    """(kept, order) of top-p filtering: probabilities sorted largest first
+    (stable), entries whose preceding mass (float32 prefix sums rounded
+    once) exceeds p set to 0, the rest renormalised (float32).
+
+    Reference: sample_top_p in scripts/profiling/profile_tcuscan_ops.py:372-392
+    (torch.sort descending, torch.cumsum, mask `probs_sum - probs_sort > p`).
+    """
+    v, order = sort(np.asarray(probs, dtype=np.float16), descending=True)
+    mass = scan(v)
+    kept = v.astype(np.float32)
+    kept = np.where(mass - kept > np.float32(p), np.float32(0), kept).astype(np.float32)
+    return kept / kept.sum(dtype=np.float64), order

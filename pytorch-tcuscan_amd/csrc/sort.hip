@@ -20,61 +20,13 @@
 // digit's run contiguously.
 #include "tcs_common.hpp"
 #include "tcs_device.hpp"
+#include "tcs_sortkey.hpp"
 
 namespace tcs {
 
-template <int DT> struct SortKey;
-template <> struct SortKey<TCS_F16> {
-  using Raw = uint16_t;
-  static constexpr int kBits = 16;
-  __device__ static uint32_t key(uint16_t r) {
-    if ((r & 0x7fffu) > 0x7c00u) return 0xffffu;  // NaN after +inf
-    if (r == 0x8000u) r = 0;                      // -0.0 ties with +0.0
-    return (r & 0x8000u) ? (uint16_t)~r : (uint32_t)(r | 0x8000u);
-  }
-};
-template <> struct SortKey<TCS_BF16> {
-  using Raw = uint16_t;
-  static constexpr int kBits = 16;
-  __device__ static uint32_t key(uint16_t r) {
-    if ((r & 0x7fffu) > 0x7f80u) return 0xffffu;
-    if (r == 0x8000u) r = 0;
-    return (r & 0x8000u) ? (uint16_t)~r : (uint32_t)(r | 0x8000u);
-  }
-};
-template <> struct SortKey<TCS_F32> {
-  using Raw = uint32_t;
-  static constexpr int kBits = 32;
-  __device__ static uint32_t key(uint32_t r) {
-    if ((r & 0x7fffffffu) > 0x7f800000u) return 0xffffffffu;
-    if (r == 0x80000000u) r = 0;
-    return (r & 0x80000000u) ? ~r : (r | 0x80000000u);
-  }
-};
-template <> struct SortKey<TCS_I8> {
-  using Raw = uint8_t;
-  static constexpr int kBits = 8;
-  __device__ static uint32_t key(uint8_t r) { return (uint32_t)(r ^ 0x80u); }
-};
-template <> struct SortKey<TCS_I16> {
-  using Raw = uint16_t;
-  static constexpr int kBits = 16;
-  __device__ static uint32_t key(uint16_t r) { return (uint32_t)(r ^ 0x8000u); }
-};
-template <> struct SortKey<TCS_I32> {
-  using Raw = uint32_t;
-  static constexpr int kBits = 32;
-  __device__ static uint32_t key(uint32_t r) { return r ^ 0x80000000u; }
-};
-template <> struct SortKey<TCS_U32> {
-  using Raw = uint32_t;
-  static constexpr int kBits = 32;
-  __device__ static uint32_t key(uint32_t r) { return r; }
-};
-
 constexpr int kSortNW = 8, kSortItems = 16;
 constexpr int64_t kSortTile = (int64_t)kSortNW * kWave * kSortItems;  // 8192
-constexpr int kHistThreads = 256, kHistMaxGroups = 1024;
+constexpr int kHistThreads = 512, kHistMaxGroups = 1024;
 constexpr int kMaxPasses = 4;
 
 // ---------------------------------------------------------------------------
@@ -92,28 +44,41 @@ __global__ __launch_bounds__(kHistThreads) void sort_hist_kernel(const typename 
   constexpr int V = 16 / (int)sizeof(Raw);
   // C copies of the counts per wave, lane l adding into copy l % C: lanes
   // whose keys share a digit (skewed data) then contend C times less.
-  constexpr int C = 8 / P;
+  constexpr int C = 4 / P > 0 ? 4 / P : 1;
   constexpr int NH = NWV * C;
-  __shared__ uint32_t h[NH][P * 256];
+  // One word of padding per copy: the copies of one bin fall in different
+  // LDS banks.
+  __shared__ uint32_t h[NH][P * 256 + 1];
   const int tid = threadIdx.x, wid = tid / kWave;
   const int hc = wid * C + (__lane_id() % C);
-  for (int k = tid; k < NH * P * 256; k += kHistThreads) (&h[0][0])[k] = 0;
+  for (int k = tid; k < NH * (P * 256 + 1); k += kHistThreads) (&h[0][0])[k] = 0;
   __syncthreads();
   const int64_t c0 = (int64_t)blockIdx.x * chunk;
   const int64_t c1 = c0 + chunk < n ? c0 + chunk : n;
-  for (int64_t b = c0 + (int64_t)tid * V; b < c1; b += (int64_t)kHistThreads * V) {
-    Raw r[V];
-    if (b + V <= c1) {
-      load_items<Raw, V, true>(x, b, c1, r);
-    } else {
-      load_items<Raw, V, false>(x, b, c1, r);
+  // Four 16-byte loads in flight per thread per step.
+  constexpr int U = 4;
+  constexpr int64_t STEP = (int64_t)kHistThreads * V;
+  for (int64_t b0 = c0 + (int64_t)tid * V; b0 < c1; b0 += U * STEP) {
+    Raw r[U][V];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t b = b0 + u * STEP;
+      if (b + V <= c1) {
+        load_items<Raw, V, true>(x, b, c1, r[u]);
+      } else {
+        load_items<Raw, V, false>(x, b, c1, r[u]);
+      }
     }
 #pragma unroll
-    for (int i = 0; i < V; ++i) {
-      if (b + i < c1) {
-        const uint32_t k = KT::key(r[i]) ^ flip;
+    for (int u = 0; u < U; ++u) {
+      const int64_t b = b0 + u * STEP;
 #pragma unroll
-        for (int p = 0; p < P; ++p) atomicAdd(&h[hc][p * 256 + ((k >> (8 * p)) & 0xffu)], 1u);
+      for (int i = 0; i < V; ++i) {
+        if (b + i < c1) {
+          const uint32_t k = KT::key(r[u][i]) ^ flip;
+#pragma unroll
+          for (int p = 0; p < P; ++p) atomicAdd(&h[hc][p * 256 + ((k >> (8 * p)) & 0xffu)], 1u);
+        }
       }
     }
   }

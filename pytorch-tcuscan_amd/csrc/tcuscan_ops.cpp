@@ -949,6 +949,72 @@ std::tuple<at::Tensor, at::Tensor> run_radix_sort_ex(const at::Tensor& x, bool d
   return sort_impl(x, descending, "run_radix_sort_ex");
 }
 
+// ---------------------------------------------------------------------------
+// Top-k (reference: src/torch/torch_topk.h:23-207)
+// ---------------------------------------------------------------------------
+
+std::tuple<at::Tensor, at::Tensor> topk_impl(const at::Tensor& x_in, int64_t k, bool largest, const char* op) {
+  require_device(x_in, op, "x");
+  const int dt = dtype_code(x_in.scalar_type());
+  TORCH_CHECK(dt >= 0 && tcs_topk_workspace_bytes(1, 1, dt) > 0, op,
+              ": x must be float16, bfloat16, float32, int8, int16, int32 or uint32, got ", x_in.scalar_type());
+  at::Tensor x = prep(x_in);
+  const int64_t n = x.numel();
+  TORCH_CHECK(n < ((int64_t)1 << 31), op, ": at most 2^31 - 1 elements");
+  TORCH_CHECK(k >= 0 && k <= n, op, ": k must be in [0, ", n, "], got ", k);
+  const at::hip::HIPGuardMasqueradingAsCUDA guard(x.device());
+  at::Tensor v = at::empty({k}, x.options());
+  at::Tensor i = at::empty({k}, x.options().dtype(at::kInt));
+  if (k > 0) {
+    auto ws = workspace(x, tcs_topk_workspace_bytes(n, k, dt));
+    at::Tensor scratch = at::empty({(int64_t)tcs_topk_scratch_bytes(n, k, dt)}, x.options().dtype(at::kByte));
+    check(tcs_topk(x.data_ptr(), dt, n, k, largest ? 1 : 0, v.data_ptr(), i.data_ptr<int32_t>(), ws.first,
+                   ws.second, scratch.data_ptr(), (size_t)scratch.numel(), stream_of(x)),
+          op);
+  }
+  return {v, i};
+}
+
+// src/torch/torch_topk.h:149-207; x_min / x_max bound the reference's value
+// bisection and are not needed by a radix select.
+std::tuple<at::Tensor, at::Tensor> run_topk_fp16(const at::Tensor& x, int64_t k, double x_min, double x_max, int S) {
+  (void)x_min;
+  (void)x_max;
+  TORCH_CHECK(S > 0, "run_topk_fp16: S must be positive");
+  TORCH_CHECK(x.scalar_type() == at::kHalf, "run_topk_fp16: x must be float16, got ", x.scalar_type());
+  return topk_impl(x, k, true, "run_topk_fp16");
+}
+
+// src/torch/torch_topk.h:69-136
+std::tuple<at::Tensor, at::Tensor> run_topk_int16(const at::Tensor& x, int64_t k, int64_t x_min, int64_t x_max,
+                                                  int S) {
+  (void)x_min;
+  (void)x_max;
+  TORCH_CHECK(S > 0, "run_topk_int16: S must be positive");
+  TORCH_CHECK(x.scalar_type() == at::kShort, "run_topk_int16: x must be int16, got ", x.scalar_type());
+  return topk_impl(x, k, true, "run_topk_int16");
+}
+
+std::tuple<at::Tensor, at::Tensor> run_topk(const at::Tensor& x, int64_t k, bool largest) {
+  return topk_impl(x, k, largest, "run_topk");
+}
+
+// src/torch/torch_topk.h:23-60: k is accepted and unused, as there.
+at::Tensor run_topk_pivot_fp16(const at::Tensor& x_in, int64_t k, int num_samples) {
+  const char* op = "run_topk_pivot_fp16";
+  (void)k;
+  require_device(x_in, op, "x");
+  TORCH_CHECK(x_in.scalar_type() == at::kHalf, op, ": x must be float16, got ", x_in.scalar_type());
+  TORCH_CHECK(num_samples >= 4 && num_samples <= 64, op, ": num_samples must be in [4, 64]");
+  TORCH_CHECK(x_in.numel() > 0, op, ": x must not be empty");
+  at::Tensor x = prep(x_in);
+  const int64_t blocks = tcs_topk_pivot_blocks(x.numel(), num_samples);
+  const at::hip::HIPGuardMasqueradingAsCUDA guard(x.device());
+  at::Tensor out = at::empty({blocks}, x.options());
+  check(tcs_topk_pivot(x.data_ptr(), x.numel(), num_samples, out.data_ptr(), blocks, stream_of(x)), op);
+  return out;
+}
+
 // src/torch/torch_compare.h:33-67: int32 [1] count of (x cmp pivot); the
 // reference's tile_len is a tiling hint with no counterpart here.
 at::Tensor run_count_if(const at::Tensor& x_in, float pivot, uint32_t tile_len, uint8_t compare_mode) {
@@ -1025,6 +1091,12 @@ PYBIND11_MODULE(tcuscan_ops, m) {
   m.def("run_where", &tcuscan::run_where, "Indices where x >= pivot (int32).");
   m.def("run_count_if", &tcuscan::run_count_if, "Count if");
   m.def("run_split", &tcuscan::run_split, "Split (stable partition by mask == 1)");
+  m.def("run_topk_fp16", &tcuscan::run_topk_fp16, "TopK (fp16): k largest values and their indices");
+  m.def("run_topk_int16", &tcuscan::run_topk_int16, "TopK (int16): k largest values and their indices");
+  m.def("run_topk", &tcuscan::run_topk, py::arg("x"), py::arg("k"), py::arg("largest") = true,
+        "TopK of any sortable dtype, largest or smallest first");
+  m.def("run_topk_pivot_fp16", &tcuscan::run_topk_pivot_fp16, py::arg("x"), py::arg("k"),
+        py::arg("num_samples") = 32, "K-largest value estimator (fp16)");
   m.def("run_radix_sort", &tcuscan::run_radix_sort, "Radix sort with indices (ascending, stable)");
   m.def("run_radix_sort_ex", &tcuscan::run_radix_sort_ex, py::arg("x"), py::arg("descending") = false,
         "Radix sort with indices, ascending or descending (stable)");

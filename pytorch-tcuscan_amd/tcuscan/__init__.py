@@ -26,4 +26,7 @@ except ImportError as exc:  # pragma: no cover - exercised only without a build
 
 from tcuscan_ops import *  # noqa: E402,F401,F403
 
-__all__ = [n for n in dir(tcuscan_ops) if n.startswith("run_")] + ["BatchPlan", "Server"]
+from .sampling import sample_top_p, top_p_filter  # noqa: E402
+
+__all__ = [n for n in dir(tcuscan_ops) if n.startswith("run_")] + ["BatchPlan", "Server", "sample_top_p",
+                                                                   "top_p_filter"]
