@@ -15,7 +15,9 @@ Printed: one JSON line with the driver's fields plus
   extras        the other workloads measured the same way: the int8 and
                 bf16 scans, run_compress_ind on N = 2^28 fp32 values with a
                 Bernoulli(0.3) mask, run_filter_greater_eq on N = 2^28 fp16
-                normals (pivot 0.5), run_searchsorted of 2^24 int32 needles
+                normals (pivot 0.5), run_radix_sort and run_topk_fp16
+                (k = 2048) on N = 2^28 fp16 normals with torch.sort /
+                torch.topk timed beside them, run_searchsorted of 2^24 int32 needles
                 into 2^24 sorted values, and run_spmv_v2 on the 2^22 x 2^22 CSR
                 matrix with lognormal row lengths (mean 16, sigma 1.2,
                 clipped to [0, 20000]; about 6.7e7 nonzeros).
@@ -515,6 +517,17 @@ def main() -> int:
             "bytes_per_step": fl["bytes_per_step"], "kept": fl["kept"],
             "ms_per_step": round(fl["wall_s"] * 1e3 / ksteps, 4),
             "note": "count pass + fused compare-and-compact pass, with the count read back to size the output",
+        }
+        so = _sort_workload(n, "fp16", max(5, ksteps // 2), 2)
+        extras[f"radix_sort_f16_2p{a.log2n}"] = {
+            "value": round(so["gkeys_per_s"], 2), "unit": "G keys/s",
+            "ms_per_step": round(so["wall_s"] * 1e3 / max(5, ksteps // 2), 4),
+            "torch_sort_stable_ms": round(so["torch_sort_ms"], 4),
+        }
+        tk = _topk_workload(n, 2048, ksteps, 2)
+        extras[f"topk_f16_2p{a.log2n}_k2048"] = {
+            "value": round(tk["ms_per_call"], 4), "unit": "ms per call",
+            "torch_topk_ms": round(tk["torch_topk_ms"], 4),
         }
         ss = _searchsorted_workload(24, 24, ksteps, max(3, a.warmup // 2))
         extras["searchsorted_i32_2p24_into_2p24"] = {
