@@ -272,6 +272,15 @@ int tcs_count_filter(const void* x, int x_dtype, double pivot, int cmp, int64_t 
 int tcs_filter(const void* x, int x_dtype, double pivot, int cmp, void* out_vals, int32_t* out_pos,
                int64_t n, int64_t out_capacity, void* workspace, size_t workspace_bytes, void* stream);
 
+/* Segmented scan from plain scans: z[i] = scan_x[i] - seg_ends[scan_f[i] - 1]
+ * (z[i] = scan_x[i] where scan_f[i] == 0), one float32 subtraction each;
+ * scan_x: inclusive scan of x, scan_f: inclusive scan of the segment-start
+ * flags, seg_ends[j]: scan_x at the last element of segment j.  Segment ids
+ * past num_seg_ends subtract nothing.  Replaces seg_scan_mc_revert behind
+ * run_seg_scan_mc_revert (src/torch/torch_seg_ops.h:108-153). */
+int tcs_seg_scan_revert(const float* scan_x, const int32_t* scan_f, const float* seg_ends, int64_t num_seg_ends,
+                        float* z, int64_t n, void* stream);
+
 /* ------------------------------------------------------------------------
  * Split (stable partition)
  * --------------------------------------------------------------------- */

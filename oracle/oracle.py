@@ -28,7 +28,7 @@ __all__ = [
     "work_distribution", "complete_rows", "complete_blocks", "seg_sum_indptr",
     "seg_sum_flags", "csr_gather", "spmv", "compress", "compress_ind",
     "compress_ind_no_arange", "out_dtype_for", "compare_mask", "count_if", "filter_cmp",
-    "searchsorted", "split", "sort", "topk", "topk_pivot", "top_p_filter",
+    "searchsorted", "split", "sort", "topk", "topk_pivot", "top_p_filter", "seg_scan_revert",
 ]
 
 _FLOAT_IN = ("float16", "bfloat16", "float32")
@@ -533,3 +533,21 @@ def top_p_filter(probs: np.ndarray, p: float):
     kept = v.astype(np.float32)
     kept = np.where(mass - kept > np.float32(p), np.float32(0), kept).astype(np.float32)
     return kept / kept.sum(dtype=np.float64), order
+
+
+def seg_scan_revert(scan_x: np.ndarray, scan_f: np.ndarray, seg_ends: np.ndarray) -> np.ndarray:
+    """z[i] = scan_x[i] - seg_ends[scan_f[i] - 1] in float32 (scan_x[i] in
+    segment 0).
+
+    Reference: run_seg_scan_mc_revert (src/torch/torch_seg_ops.h:108-153);
+    tests/test_seg_scan_mc_revert.py builds scan_x = cumsum(x), scan_f =
+    cumsum(f) and seg_ends = scan_x at every segment's last element, and
+    compares with a per-segment cumsum.
+    """
+    sx = np.asarray(scan_x, dtype=np.float32).reshape(-1)
+    sf = np.asarray(scan_f).reshape(-1).astype(np.int64)
+    d = np.asarray(seg_ends, dtype=np.float32).reshape(-1)
+    j = sf - 1
+    ok = (j >= 0) & (j < d.size)
+    sub = np.where(ok, d[np.clip(j, 0, max(d.size - 1, 0))] if d.size else 0, 0).astype(np.float32)
+    return (sx - sub).astype(np.float32)

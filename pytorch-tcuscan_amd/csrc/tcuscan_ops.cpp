@@ -520,6 +520,41 @@ at::Tensor run_spmv_v2(const at::Tensor& vals_in, const at::Tensor& indptr_in,
   return z;
 }
 
+// src/torch/torch_spmv.h:235-302: fp16 vals and x, int32 indptr; S is the
+// side of `upper` (the tiling hint of the reference's block scan).
+at::Tensor run_spmv_v2_multi_cube(const at::Tensor& vals, const at::Tensor& indptr, const at::Tensor& cols,
+                                  const at::Tensor& x, const at::Tensor& upper, const at::Tensor& lower_strict) {
+  const char* op = "run_spmv_v2_multi_cube";
+  TORCH_CHECK(vals.scalar_type() == at::kHalf && x.scalar_type() == at::kHalf, op,
+              ": vals and x must both be fp16, got vals=", vals.scalar_type(), " x=", x.scalar_type());
+  TORCH_CHECK(indptr.scalar_type() == at::kInt, op, ": indptr must be int32, got ", indptr.scalar_type());
+  TORCH_CHECK(upper.dim() == 2 && upper.size(0) == upper.size(1) && lower_strict.sizes() == upper.sizes(), op,
+              ": upper / lower_strict must be S x S");
+  return run_spmv_v2(vals, indptr, cols, x, (int)upper.size(0), c10::nullopt);
+}
+
+// src/torch/torch_seg_ops.h:108-153: z[i] = scan_x[i] - diff[scan_f[i] - 1]
+// (scan_x in segment 0), float32.
+at::Tensor run_seg_scan_mc_revert(const at::Tensor& scan_x_in, const at::Tensor& scan_f_in,
+                                  const at::Tensor& diff_in) {
+  const char* op = "run_seg_scan_mc_revert";
+  require_device(scan_x_in, op, "x");
+  require_device(scan_f_in, op, "f");
+  require_device(diff_in, op, "diff");
+  TORCH_CHECK(scan_x_in.scalar_type() == at::kFloat, op, ": x must be float32, got ", scan_x_in.scalar_type());
+  TORCH_CHECK(scan_f_in.scalar_type() == at::kInt, op, ": f must be int32, got ", scan_f_in.scalar_type());
+  TORCH_CHECK(diff_in.scalar_type() == at::kFloat, op, ": diff must be float32, got ", diff_in.scalar_type());
+  TORCH_CHECK(scan_f_in.numel() == scan_x_in.numel(), op, ": x and f must have the same length");
+  at::Tensor sx = prep(scan_x_in), sf = prep(scan_f_in), d = diff_in.contiguous();
+  const int64_t n = sx.numel();
+  const at::hip::HIPGuardMasqueradingAsCUDA guard(sx.device());
+  at::Tensor z = at::empty({n}, sx.options());
+  check(tcs_seg_scan_revert(sx.data_ptr<float>(), sf.data_ptr<int32_t>(), d.numel() ? d.data_ptr<float>() : nullptr,
+                            d.numel(), z.data_ptr<float>(), n, stream_of(sx)),
+        op);
+  return z;
+}
+
 // src/torch/torch_gather.h:92-172
 at::Tensor run_csr_gather(const at::Tensor& values_in, const at::Tensor& cols_in, const at::Tensor& x_in) {
   const char* op = "run_csr_gather";
@@ -1125,6 +1160,10 @@ PYBIND11_MODULE(tcuscan_ops, m) {
         py::arg("x"), py::arg("s"), py::arg("segm_offsets") = py::none(),
         "Sparse Matrix-Vector Multiplication Using Segmented Sum");
   m.def("run_csr_gather", &tcuscan::run_csr_gather, "CSR gather");
+  m.def("run_spmv_v2_multi_cube", &tcuscan::run_spmv_v2_multi_cube, py::arg("vals"), py::arg("indptr"),
+        py::arg("cols"), py::arg("x"), py::arg("upper"), py::arg("lower_strict"),
+        "SpMV with the multi-cube block scan (fp16)");
+  m.def("run_seg_scan_mc_revert", &tcuscan::run_seg_scan_mc_revert, "Vector Revert for MC Segmented Scan");
   m.def("run_compress", &tcuscan::run_compress, "Compaction/compress");
   m.def("run_compress_pos", &tcuscan::run_compress_pos,
         "Compaction/compress with pre-computed output positions");
