@@ -30,3 +30,18 @@ def assert_exact(actual: np.ndarray, expected: np.ndarray, what: str = ""):
     assert bad.size == 0, (
         f"{what}: {bad.size} mismatches, first at {bad[:5].tolist()}: "
         f"{actual[bad[:5]].tolist()} vs {expected[bad[:5]].tolist()}")
+
+
+def sparse_random(nrow: int, ncol: int, density: float, seed: int):
+    """CSR matrix with round(density * nrow * ncol) distinct uniformly random
+    positions and values uniform in [-1, 1) (scipy.sparse.random's shape and
+    density; its legacy sampler permutes all nrow * ncol positions, which
+    takes seconds at 16K x 16K)."""
+    import scipy.sparse as sp
+    rng = np.random.default_rng(seed)
+    k = int(round(density * nrow * ncol))
+    pos = rng.choice(nrow * ncol, size=k, replace=False)
+    data = 2 * rng.random(k) - 1
+    m = sp.csr_matrix((data, (pos // ncol, pos % ncol)), shape=(nrow, ncol))
+    m.sort_indices()
+    return m

@@ -10,11 +10,10 @@ run_spmv_v2, and rejecting the dtypes the reference rejects.
 """
 import numpy as np
 import pytest
-import scipy.sparse as sp
 import torch
 
 import oracle
-from _util import assert_exact, to_np
+from _util import assert_exact, sparse_random, to_np
 
 MULT = [1, 2, 3, 5, 11, 23]  # tests/test_seg_scan_mc_revert.py:35
 NROW = [64 * 64 - 1, 24 * 128 + 1, 16 * 128 - 13, 64 * 128 + 13, 32 * 128 - 133]  # tests/test_spmv_v2_multi_cube.py:30
@@ -84,9 +83,7 @@ def test_spmv_v2_multi_cube_reference_grid(ops, dev, s, density):
     ones = torch.ones((s, s), dtype=torch.float16, device=dev)
     upper, lower = torch.triu(ones), torch.tril(ones, -1)
     for nrow in NROW:
-        rs = np.random.RandomState(nrow + s)
-        B = sp.random(nrow - 1, nrow - 1, density=density, format="csr", random_state=rs,
-                      data_rvs=lambda k: 2 * rs.uniform(0, 1, size=k) - 1)
+        B = sparse_random(nrow - 1, nrow - 1, density, nrow + s)
         vals = B.data.astype(np.float16)
         ip = B.indptr.astype(np.int32)
         cols = B.indices.astype(np.int32)

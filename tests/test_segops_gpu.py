@@ -18,7 +18,7 @@ import scipy.sparse as sp
 import torch
 
 import oracle
-from _util import assert_exact, gen, to_np
+from _util import assert_exact, gen, sparse_random, to_np
 
 pytestmark = pytest.mark.gpu
 
@@ -284,9 +284,7 @@ def test_csr_gather_rejects_bad_args(ops, dev):
 @pytest.mark.parametrize("nrow", [1024, 1024 * 3, 1024 * 8, 1024 * 16])
 @pytest.mark.parametrize("dtype", [torch.float32, torch.float16], ids=str)
 def test_spmv_v2(ops, dev, s, density, nrow, dtype):
-    rs = np.random.RandomState(nrow + int(1 / density))
-    B = sp.random(nrow - 1, nrow - 1, density=density, format="csr", random_state=rs,
-                  data_rvs=lambda k: 2 * rs.uniform(0, 1, size=k) - 1)
+    B = sparse_random(nrow - 1, nrow - 1, density, nrow + int(1 / density))
     vec = np.random.default_rng(42).uniform(1, 9, nrow - 1)
     vals = torch.from_numpy(B.data.astype(np.float32)).to(dtype)
     xv = torch.from_numpy(vec.astype(np.float32)).to(dtype)
