@@ -193,7 +193,7 @@ __global__ __launch_bounds__(kCmpNW * kWave) void compress_kernel(
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    for (int k = lane; k < wn; k += kWave) out[base + k] = (OT)stage[k];
+    for (int k = lane; k < wn; k += kWave) __builtin_nontemporal_store((OT)stage[k], out + base + k);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");

@@ -356,9 +356,11 @@ __device__ __forceinline__ void tile_output(TileRegs<C, J>& r, const typename C:
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      // Whole lines, written once and not read again by this kernel:
+      // nontemporal stores (5.60 -> 5.88 TB/s fp16, 5.44 -> 6.56 int8 at 2^28).
       Out* gy = y + (wbase - (int64_t)lane * G) + j * Gm::GSPAN;
-      *reinterpret_cast<u32x4*>(gy + lane * 4) = w0;
-      *reinterpret_cast<u32x4*>(gy + kWave * 4 + lane * 4) = w1;
+      __builtin_nontemporal_store(w0, reinterpret_cast<u32x4*>(gy + lane * 4));
+      __builtin_nontemporal_store(w1, reinterpret_cast<u32x4*>(gy + kWave * 4 + lane * 4));
     } else {
       store_items<Out, G, FULL>(y, wbase + j * Gm::GSPAN, n, o);
     }
@@ -421,7 +423,8 @@ __device__ __forceinline__ void tile_output_mfma(TileRegs<C, J>& r, const PipeSl
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 #pragma unroll
-      for (int k = 0; k < 4; ++k) *reinterpret_cast<u32x4*>(y + ub + k * kWave * 4 + lane * 4) = w[k];
+      for (int k = 0; k < 4; ++k)
+        __builtin_nontemporal_store(w[k], reinterpret_cast<u32x4*>(y + ub + k * kWave * 4 + lane * 4));
     } else {
 #pragma unroll
       for (int q = 0; q < 4; ++q) {

@@ -147,8 +147,8 @@ __global__ __launch_bounds__(kSplitNW * kWave) void split_kernel(
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   for (int k = lane; k < wvalid; k += kWave) {
     const int64_t dst = k < wtot ? kbase + k : rbase + (k - wtot);
-    out_x[dst] = stage[k];
-    if constexpr (IDX) out_idx[dst] = s_istage[wid][k];
+    __builtin_nontemporal_store(stage[k], out_x + dst);
+    if constexpr (IDX) __builtin_nontemporal_store(s_istage[wid][k], out_idx + dst);
   }
 }
 
