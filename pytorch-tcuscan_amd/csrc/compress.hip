@@ -24,9 +24,9 @@
 namespace tcs {
 
 constexpr int kCmpNW = 8, kCmpG = 8;
-// The smallest tile of any compaction (2 groups per wave); workspaces are
+// The smallest tile of any compaction (1 group per wave); workspaces are
 // sized for this many tiles.
-constexpr int64_t kCmpTile = (int64_t)kCmpNW * 2 * kWave * kCmpG;  // 8192
+constexpr int64_t kCmpTile = (int64_t)kCmpNW * 1 * kWave * kCmpG;  // 4096
 
 template <int BYTES> struct Bits;
 template <> struct Bits<1> { using T = uint8_t; };
@@ -85,7 +85,7 @@ struct CmpGeom {
   static constexpr int kByLds = 16 / kStageBytes;
   static constexpr int J = kByInput < kByLds ? kByInput : kByLds;
   static constexpr int64_t kTile = (int64_t)kCmpNW * J * kWave * kCmpG;
-  static_assert(kTile >= kCmpTile, "workspaces are sized for 8192-element tiles");
+  static_assert(kTile >= kCmpTile, "workspaces are sized for 4096-element tiles");
 };
 
 template <int XB, int IB, int IDX, class Keep>
@@ -227,6 +227,266 @@ __global__ __launch_bounds__(kCmpNW * kWave) void compress_kernel(
   }
 }
 
+// Filters (a comparison with x decides what is kept) run persistently.
+// Tile geometry: NW waves x J groups x 64 lanes x G items, about 16-36 KiB
+// of input per tile; the kernel holds two tiles in registers and one
+// wave's kept elements of one tile in LDS (at most 8 KiB per wave).  Mask
+// compactions measured faster with one tile per workgroup
+// (compress_kernel): 0.68 vs 0.78 ms for run_compress_ind at 2^28, while
+// the fp16 filter runs 0.32 ms here vs 0.385 there.
+template <int XB, int IB, int IDX, bool FROMX>
+struct FltGeom {
+  static constexpr int kInBytes = XB + (FROMX ? 0 : 1) + (IDX == 1 ? IB : 0);
+  static constexpr int kStageBytes = IDX == 0 ? XB : (IDX == 4 ? IB : (XB > IB ? XB : IB));
+  static constexpr int kByInput = kInBytes <= 2 ? 4 : (kInBytes <= 9 ? 2 : 1);
+  static constexpr int kByLds = 16 / kStageBytes;
+  static constexpr int J = kByInput < kByLds ? kByInput : kByLds;
+  static constexpr int64_t kTile = (int64_t)kCmpNW * J * kWave * kCmpG;
+  static_assert(kTile >= kCmpTile, "workspaces are sized for 4096-element tiles");
+};
+
+// One tile as a data wave holds it: values, kept bits, the mask bytes or
+// the index entries it came with.
+template <class XT, class IT, int J, int G, bool MASK, bool IDX>
+struct CmpRegs {
+  Packed<XT, G> xv[J];
+  Packed<int8_t, G> mk[MASK ? J : 1];
+  Packed<IT, G> iv[IDX ? J : 1];
+  uint32_t mb[J];  // bit i: item i of group j is kept
+};
+
+// LDS state of a tile in flight (two alternate).
+template <int NW>
+struct CmpSlot {
+  int wval[NW];   // kept per data wave
+  int wexcl[NW];  // kept in the waves before
+  int agg;        // kept in the tile
+  int carry;      // kept in the tiles before
+  int count;      // data waves that have counted the tile
+};
+
+// Persistent filter compaction with NW data waves and one look-back wave.
+// Workgroups draw tiles from the ticket counters of the workspace header
+// exactly as the scan does (workgroup b draws from counter b % K, which
+// hands out tiles k, k + K, ...; one draw ahead), so every tile below the
+// smallest undrawn one is held by a running workgroup or done.  One step
+// per tile t, with the next tile tn going into the other register set:
+// while the look-back wave walks back for t's carry (and publishes t's
+// inclusive count), the data waves load and count tn and the last of them
+// to finish publishes tn's aggregate at once (an LDS counter, no
+// barrier); after one barrier the data waves stage t's kept elements in
+// LDS in output order and write them with coalesced stores.
+template <int XB, int IB, int IDX, class Keep>
+__global__ __launch_bounds__((kCmpNW + 1) * kWave) void filter_kernel(
+    const typename Bits<XB>::T* __restrict__ x, const typename Bits<IB>::T* __restrict__ idx_in,
+    Keep keep, typename Bits<XB>::T* __restrict__ out_x,
+    typename Bits<IB>::T* __restrict__ out_idx, int64_t n, int64_t cap, void* ws, uint32_t num_tiles) {
+  using XT = typename Bits<XB>::T;
+  using IT = typename Bits<IB>::T;
+  using Geom = FltGeom<XB, IB, IDX, Keep::kFromX>;
+  constexpr int G = kCmpG, J = Geom::J, NW = kCmpNW;
+  constexpr bool kMask = !Keep::kFromX;
+  constexpr int64_t TILE = Geom::kTile;
+  constexpr int64_t GSPAN = (int64_t)kWave * G, WSEG = GSPAN * J;
+  using ST = typename Bits<Geom::kStageBytes>::T;
+  using Regs = CmpRegs<XT, IT, J, G, kMask, IDX == 1>;
+  __shared__ CmpSlot<NW> slot[2];
+  __shared__ uint32_t s_draw[2];
+  __shared__ uint32_t s_tag;
+  __shared__ ST s_stage[NW][J * G * kWave];  // one wave's kept elements, in order
+
+  const int lane = __lane_id();
+  const int wid = threadIdx.x / kWave;  // wave NW is the look-back wave
+  const bool lb_wave = wid == NW;
+  const uint32_t T = num_tiles;
+  const uint32_t K = gridDim.x < (uint32_t)kTicketCounters ? gridDim.x : (uint32_t)kTicketCounters;
+  const uint32_t grp = blockIdx.x % K;
+  uint64_t* ctr = ws_slot(ws, (int)grp);
+  const uint64_t grp_tiles = ((uint64_t)T - grp + K - 1) / K;
+  const uint64_t grp_wgs = (gridDim.x - grp + K - 1) / K;
+  const uint64_t last_draw = grp_tiles + grp_wgs - 1;
+  auto draw = [&]() -> uint32_t {
+    const uint64_t v = __hip_atomic_fetch_add(ctr, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (v == last_draw) __hip_atomic_store(ctr, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return v >= grp_tiles ? T : (uint32_t)(v * K + grp);
+  };
+
+  auto load = [&](Regs& r, uint32_t t) {
+    const int64_t t0 = (int64_t)t * TILE;
+    const int64_t wbase = t0 + (int64_t)wid * WSEG + (int64_t)lane * G;
+    if (t0 + TILE <= n) {
+#pragma unroll
+      for (int j = 0; j < J; ++j) {
+        const int64_t gb = wbase + j * GSPAN;
+        if constexpr (kMask) load_items<int8_t, G, true>(keep.mask, gb, n, r.mk[j]);
+        load_items<XT, G, true>(x, gb, n, r.xv[j]);
+        if constexpr (IDX == 1) load_items<IT, G, true>(idx_in, gb, n, r.iv[j]);
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < J; ++j) {
+        const int64_t gb = wbase + j * GSPAN;
+        if constexpr (kMask) load_items<int8_t, G, false>(keep.mask, gb, n, r.mk[j]);
+        load_items<XT, G, false>(x, gb, n, r.xv[j]);
+        if constexpr (IDX == 1) load_items<IT, G, false>(idx_in, gb, n, r.iv[j]);
+      }
+    }
+  };
+  // The last data wave to count a tile: the waves' exclusive offsets and
+  // the tile's aggregate.
+  auto finalize = [&](CmpSlot<NW>& sl, uint32_t t) {
+    const int a = lane < NW ? sl.wval[lane] : 0;
+    const int incl = wave_incl_scan(a);
+    const int agg = readlane(incl, NW - 1);
+    if (lane < NW) sl.wexcl[lane] = incl - a;
+    if (lane == 0) {
+      sl.agg = agg;
+      sl.count = 0;
+      uint64_t* desc = ws_desc(ws);
+      if (t == 0) {
+        if (T > 1) desc_publish<int>(desc, 0, agg, s_tag, kFlagIncl);
+      } else {
+        desc_publish<int>(desc, t, agg, s_tag, kFlagAgg);
+      }
+    }
+  };
+  // Kept bits, per-group exclusive offsets inside the wave, wave total.
+  auto count = [&](Regs& r, int (&gex)[J], CmpSlot<NW>& sl, uint32_t t) {
+    const int64_t wbase = (int64_t)t * TILE + (int64_t)wid * WSEG + (int64_t)lane * G;
+    int wtot = 0;
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+      const int64_t gb = wbase + j * GSPAN;
+      uint32_t b = 0;
+#pragma unroll
+      for (int i = 0; i < G; ++i) {
+        bool k;
+        if constexpr (kMask) {
+          k = r.mk[j][i] == 1;  // items past n load as 0
+        } else {
+          k = gb + i < n && keep.of(r.xv[j][i]);
+        }
+        b |= (k ? 1u : 0u) << i;
+      }
+      r.mb[j] = b;
+      const int c = __builtin_popcount(b);
+      const int incl = wave_incl_scan(c);
+      gex[j] = wtot + incl - c;
+      wtot += readlane(incl, kWave - 1);
+    }
+    int last = 0;
+    if (lane == 0) {
+      sl.wval[wid] = wtot;
+      last = __hip_atomic_fetch_add(&sl.count, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP) == NW - 1;
+    }
+    if (readlane(last, 0)) finalize(sl, t);
+  };
+  // Look-back wave: kept elements in the tiles before t.  Counts are 32-bit:
+  // n < 2^31 per call (checked on the host).
+  auto walk = [&](CmpSlot<NW>& sl, uint32_t t) {
+    auto fb = [&](int64_t u) {
+      const int64_t u0 = u * TILE;
+      const int64_t u1 = u0 + TILE < n ? u0 + TILE : n;
+      int c = 0;
+      for (int64_t p = u0 + lane; p < u1; p += kWave) c += keep.at(x, p) ? 1 : 0;
+      return FlagVal<int>{u == 0 ? 1 : 0, wave_sum(c)};
+    };
+    int carry = 0;
+    if (t > 0) {
+      carry = lookback<int>(ws_desc(ws), t, s_tag, fb);
+      if (lane == 0) desc_publish<int>(ws_desc(ws), t, carry + sl.agg, s_tag, kFlagIncl);
+    }
+    if (lane == 0) sl.carry = carry;
+  };
+  // Stage this wave's kept elements of t in output order, then write them.
+  auto write = [&](const Regs& r, const int (&gex)[J], const CmpSlot<NW>& sl, uint32_t t) {
+    const int64_t base = (int64_t)sl.carry + sl.wexcl[wid];
+    const int wtot = sl.wval[wid];
+    const int64_t wbase = (int64_t)t * TILE + (int64_t)wid * WSEG + (int64_t)lane * G;
+    ST* stage = s_stage[wid];
+    const int64_t room = cap - base;
+    const int wn = room <= 0 ? 0 : (room < wtot ? (int)room : wtot);
+    auto flush = [&](auto* out) {
+      using OT = typename std::remove_pointer<decltype(out)>::type;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      for (int k = lane; k < wn; k += kWave) __builtin_nontemporal_store((OT)stage[k], out + base + k);
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    };
+    if constexpr (IDX != 4) {
+#pragma unroll
+      for (int j = 0; j < J; ++j) {
+        int pos = gex[j];
+#pragma unroll
+        for (int i = 0; i < G; ++i)
+          if ((r.mb[j] >> i) & 1u) stage[pos++] = (ST)r.xv[j][i];
+      }
+      flush(out_x);
+    }
+    if constexpr (IDX != 0) {
+#pragma unroll
+      for (int j = 0; j < J; ++j) {
+        int pos = gex[j];
+        const int64_t gb = wbase + j * GSPAN;
+#pragma unroll
+        for (int i = 0; i < G; ++i) {
+          if ((r.mb[j] >> i) & 1u) {
+            if constexpr (IDX == 1) stage[pos] = (ST)r.iv[j][i];
+            if constexpr (IDX == 2 || IDX == 4) stage[pos] = (ST)(IT)(gb + i);
+            if constexpr (IDX == 3) stage[pos] = (ST)(IT)(gb + i + 1);
+            ++pos;
+          }
+        }
+      }
+      flush(out_idx);
+    }
+  };
+
+  if (threadIdx.x < 2) slot[threadIdx.x].count = 0;
+  if (lb_wave && lane == 0) {
+    s_tag = launch_arrive(ws);
+    const uint32_t d0 = draw();
+    s_draw[0] = d0;
+    s_draw[1] = d0 < T ? draw() : T;
+  }
+  __syncthreads();
+  uint32_t t = s_draw[0];
+  uint32_t tn = s_draw[1];
+  if (t >= T) return;
+
+  Regs reg[2];
+  int gex[2][J];
+  if (!lb_wave) {
+    load(reg[0], t);
+    count(reg[0], gex[0], slot[0], t);
+  }
+  __syncthreads();
+
+  auto step = [&](auto parity) -> bool {
+    constexpr int P = decltype(parity)::value;
+    constexpr int Q = 1 - P;
+    if (lb_wave) {
+      walk(slot[P], t);
+      if (lane == 0) s_draw[P] = tn < T ? draw() : T;
+    } else if (tn < T) {
+      load(reg[Q], tn);
+      count(reg[Q], gex[Q], slot[Q], tn);
+    }
+    __syncthreads();
+    if (!lb_wave) write(reg[P], gex[P], slot[P], t);
+    t = tn;
+    tn = s_draw[P];
+    return t < T;
+  };
+  for (;;) {
+    if (!step(std::integral_constant<int, 0>{})) break;
+    if (!step(std::integral_constant<int, 1>{})) break;
+  }
+}
+
 // Count of mask == 1: per-workgroup partial counts, then one workgroup sums
 // them in a fixed order.
 constexpr int kCountBlock = 256;
@@ -336,15 +596,39 @@ template <int XB, int IB, int IDX, class Keep = KeepMask>
 static int launch_compress(const void* x, const void* idx_in, Keep keep, void* out_x,
                            void* out_idx, int64_t n, int64_t cap, void* ws, uint32_t,
                            hipStream_t st) {
+  using XT = typename Bits<XB>::T;
+  using IT = typename Bits<IB>::T;
   // The workspace holds descriptors for n / kCmpTile tiles, at least as many
-  // as this instantiation's larger tiles need.
-  constexpr int64_t kTile = CmpGeom<XB, IB, IDX, Keep::kFromX>::kTile;
-  const uint32_t tiles = (uint32_t)((n + kTile - 1) / kTile);
-  hipLaunchKernelGGL((compress_kernel<XB, IB, IDX, Keep>), dim3(tiles), dim3(kCmpNW * kWave), 0, st,
-                     static_cast<const typename Bits<XB>::T*>(x),
-                     static_cast<const typename Bits<IB>::T*>(idx_in), keep,
-                     static_cast<typename Bits<XB>::T*>(out_x),
-                     static_cast<typename Bits<IB>::T*>(out_idx), n, cap, ws, tiles);
+  // as either kernel's tiles need.
+  if constexpr (!Keep::kFromX) {
+    constexpr int64_t kTile = CmpGeom<XB, IB, IDX, false>::kTile;
+    const uint32_t tiles = (uint32_t)((n + kTile - 1) / kTile);
+    hipLaunchKernelGGL((compress_kernel<XB, IB, IDX, Keep>), dim3(tiles), dim3(kCmpNW * kWave), 0, st,
+                       static_cast<const XT*>(x), static_cast<const IT*>(idx_in), keep, static_cast<XT*>(out_x),
+                       static_cast<IT*>(out_idx), n, cap, ws, tiles);
+  } else {
+    // Persistent grid: the resident workgroups, at most one per tile.
+    constexpr int64_t kTile = FltGeom<XB, IB, IDX, true>::kTile;
+    const uint32_t tiles = (uint32_t)((n + kTile - 1) / kTile);
+    static int per_cu[16] = {0};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 16) dev = 0;
+    if (per_cu[dev] == 0) {
+      int b = 0;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, filter_kernel<XB, IB, IDX, Keep>, (kCmpNW + 1) * kWave,
+                                                       0) != hipSuccess ||
+          b < 1)
+        b = 1;
+      per_cu[dev] = b;
+    }
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 256;
+    uint32_t grid = (uint32_t)per_cu[dev] * (uint32_t)cus;
+    if (grid > tiles) grid = tiles;
+    hipLaunchKernelGGL((filter_kernel<XB, IB, IDX, Keep>), dim3(grid), dim3((kCmpNW + 1) * kWave), 0, st,
+                       static_cast<const XT*>(x), static_cast<const IT*>(idx_in), keep, static_cast<XT*>(out_x),
+                       static_cast<IT*>(out_idx), n, cap, ws, tiles);
+  }
   TCS_CHECK_LAUNCH();
   return TCS_OK;
 }
