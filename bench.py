@@ -12,7 +12,8 @@ Printed: one JSON line with the driver's fields plus
   cpu_baseline  the oracle's C port of the reference's run_scan_cpu
                 (three-phase OpenMP blocked scan), timed on this host over
                 whole copies of the same workload;
-  extras        the other workloads measured the same way: the int8 and
+  extras        the other workloads measured the same way: the N = 2^20
+                fp16 scan's per-call latency (run_scan_single_core), the int8 and
                 bf16 scans, run_compress_ind on N = 2^28 fp32 values with a
                 Bernoulli(0.3) mask, run_filter_greater_eq on N = 2^28 fp16
                 normals (pivot 0.5), run_radix_sort and run_topk_fp16
@@ -126,6 +127,34 @@ def _compress_workload(n: int, S: int, steps: int, warmup: int):
     torch.cuda.empty_cache()
     return {"bytes_per_step": bytes_per_step, "wall_s": wall, "event_avg_ms": sum(ev_ms) / len(ev_ms),
             "gbs_wall": bytes_per_step * steps / wall / 1e9, "kept": k}
+
+
+def _latency_workload(log2n: int = 20, calls: int = 2000):
+    """run_scan_single_core(x, 128) on N = 2^20 fp16 U[-1, 1] (seed 0): the
+    per-call latency from the host's view (launch, kernel, and the caller
+    waiting for the result), p50 / p99 over `calls` synchronised calls, and
+    the kernel-only rate from back-to-back calls."""
+    import torch
+
+    import tcuscan
+
+    dev = torch.device("cuda:0")
+    n = 1 << log2n
+    g = torch.Generator(device="cpu").manual_seed(0)
+    x = (torch.rand(n, generator=g) * 2 - 1).half().to(dev)
+    for _ in range(200):
+        tcuscan.run_scan_single_core(x, 128)
+    torch.cuda.synchronize()
+    lat = []
+    for _ in range(calls):
+        t0 = time.perf_counter()
+        tcuscan.run_scan_single_core(x, 128)
+        torch.cuda.synchronize()
+        lat.append(time.perf_counter() - t0)
+    lat.sort()
+    wall, ev = _timed_steps(lambda: tcuscan.run_scan_single_core(x, 128), 500, 50, spin_s=0.1)
+    return {"p50_us": lat[len(lat) // 2] * 1e6, "p99_us": lat[int(0.99 * len(lat))] * 1e6,
+            "back_to_back_us": wall / 500 * 1e6, "gbs_back_to_back": 6 * n * 500 / wall / 1e9}
 
 
 def _filter_workload(n: int, S: int, steps: int, warmup: int, pivot: float = 0.5):
@@ -510,6 +539,13 @@ def main() -> int:
             "bytes_per_step": c["bytes_per_step"], "kept": c["kept"],
             "ms_per_step": round(c["wall_s"] * 1e3 / ksteps, 4),
             "note": "includes the device-to-host count read that sizes the outputs",
+        }
+        lt = _latency_workload()
+        extras["scan_f16_2p20_single_core_latency"] = {
+            "value": round(lt["p50_us"], 2), "unit": "us p50 per synchronised call",
+            "p99_us": round(lt["p99_us"], 2), "back_to_back_us": round(lt["back_to_back_us"], 2),
+            "gbs_back_to_back": round(lt["gbs_back_to_back"], 1),
+            "note": "N = 2^20 fp16 -> fp32, S = 128; the 6 MiB working set sits in the caches",
         }
         fl = _filter_workload(n, a.S, ksteps, max(3, a.warmup // 2))
         extras[f"filter_ge_f16_2p{a.log2n}_pivot0.5"] = {

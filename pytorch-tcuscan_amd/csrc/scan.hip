@@ -7,9 +7,11 @@ extern "C" {
 
 size_t tcs_scan_workspace_bytes(int64_t n, int S, int tiles_per_wg) {
   if (n <= 0 || S <= 0) return tcs::lookback_ws_bytes(0);
-  // Sized for the smallest tile any dtype uses with these parameters.
+  // Sized for the smallest tile any dtype uses with these parameters (the
+  // default tile of short inputs is 8K).
   int64_t tile = tcs::tile_elems_for(S, tiles_per_wg);
   if (tile > 16384) tile = 16384;
+  if (tiles_per_wg <= 0 && tile > 8192) tile = 8192;
   return tcs::lookback_ws_bytes((uint64_t)((n + tile - 1) / tile));
 }
 
