@@ -1,5 +1,6 @@
 """Writes tests/golden/small_cases.json: known-answer cases for the scan
-family, segmented sums, SpMV, gather and compaction.
+family, segmented sums, SpMV, gather, compaction and the consumers of scan
+(filters, count_if, split, sort, top-k, searchsorted, segmented-scan revert).
 
 The reference ships no golden vectors (its tests draw seeded random data on
 the device), so these fixtures are built here from first principles:
@@ -214,6 +215,68 @@ def main():
             carry = row[-1]
     cases["complete_rows_f32"] = {"x_f32": xs, "sums_f32": sums, "tile_width": w, "tile_height": h,
                                   "y_f32": [f32_bits(v) for v in y]}
+
+    # --- Consumers of scan: filters, count_if, split, sort, top-k, search,
+    # the segmented-scan revert.  Values compared as exact fractions, NaN
+    # first tested separately; ties broken by input position.
+    def f16_is_nan(b):
+        return (b & 0x7FFF) > 0x7C00
+
+    x = rand_f16(rng, 300)
+    p16 = int(np.array([np.float32(0.1)]).astype(np.float16).view(np.uint16)[0])  # pivot 0.1 as half
+    for k in range(0, 300, 23):
+        x[k] = p16  # ties at the pivot
+    x[5], x[17], x[29], x[41] = 0x7E00, 0x7C00, 0xFC00, 0x8000  # NaN, +inf, -inf, -0
+    fp = f16_bits_to_frac(p16)
+
+    def fv(b):  # exact value (inf as +-2^20, beyond every finite half)
+        if (b & 0x7FFF) == 0x7C00:
+            return Fraction(2 ** 20) * (-1 if b & 0x8000 else 1)
+        return f16_bits_to_frac(b)
+    ge = [b for b in x if not f16_is_nan(b) and fv(b) >= fp]
+    le = [b for b in x if not f16_is_nan(b) and fv(b) <= fp]
+    pos = [i for i, b in enumerate(x) if not f16_is_nan(b) and fv(b) >= fp]
+    cmps = [lambda a: a < fp, lambda a: a > fp, lambda a: a == fp, lambda a: a <= fp, lambda a: a >= fp,
+            lambda a: a != fp]
+    counts = [sum(1 for b in x if (f16_is_nan(b) and m == 5) or (not f16_is_nan(b) and cmps[m](fv(b))))
+              for m in range(6)]
+    cases["filters_f16"] = {"x_f16": x, "pivot": 0.1, "ge_f16": ge, "le_f16": le, "where": pos,
+                            "count_if": counts}
+
+    xi = [rng.randint(-300, 300) for _ in range(257)]
+    m = [rng.choice([0, 1, 1, 2, -1]) for _ in range(257)]
+    order = [i for i in range(257) if m[i] == 1] + [i for i in range(257) if m[i] != 1]
+    cases["split_i16"] = {"x_i16": xi, "mask": m, "z_i16": [xi[i] for i in order],
+                          "idx_out": [i + 1 for i in order]}  # indices_in = 1..n
+
+    def sort_key(b, i):  # NaN last, -0 == +0, ties by position
+        return (1, Fraction(0), i) if f16_is_nan(b) else (0, fv(b), i)
+    asc = sorted(range(300), key=lambda i: sort_key(x[i], i))
+    desc = sorted(range(300), key=lambda i: (0 if f16_is_nan(x[i]) else 1,
+                                              Fraction(0) if f16_is_nan(x[i]) else -fv(x[i]), i))
+    cases["sort_f16"] = {"x_f16": x, "asc_idx": asc, "desc_idx": desc}
+    ys = [rng.randint(-20, 20) for _ in range(400)]
+    cases["sort_i16"] = {"x_i16": ys, "asc_idx": sorted(range(400), key=lambda i: (ys[i], i)),
+                         "desc_idx": sorted(range(400), key=lambda i: (-ys[i], i))}
+    cases["topk_f16"] = {"x_f16": x, "k": 7, "largest_idx": desc[:7], "smallest_idx": asc[:7]}
+
+    hay = sorted(rng.randint(-50, 50) for _ in range(200))
+    needles = [rng.randint(-60, 60) for _ in range(100)] + [-2 ** 31, 2 ** 31 - 1]
+    cases["searchsorted_i32"] = {"sorted": hay, "values": needles,
+                                 "out": [sum(1 for h in hay if h < v) for v in needles]}
+
+    xr = [rng.randint(0, 9) for _ in range(400)]
+    fr = [0] + [1 if rng.random() < 0.05 else 0 for _ in range(399)]
+    sx = [sum(xr[:i + 1]) for i in range(400)]
+    sf = [sum(fr[:i + 1]) for i in range(400)]
+    ends = [sx[i] for i in range(400) if i == 399 or fr[i + 1] == 1]
+    seg = []
+    for i in range(400):
+        if i == 0 or fr[i]:
+            run = 0
+        run += xr[i]
+        seg.append(run)
+    cases["seg_scan_revert_f32"] = {"scan_x": sx, "scan_f": sf, "seg_ends": ends, "z": seg}
 
     with open(os.path.join(HERE, "small_cases.json"), "w") as fh:
         json.dump(cases, fh, separators=(",", ":"))

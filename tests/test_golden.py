@@ -1,7 +1,10 @@
 """Known-answer fixtures (tests/golden/small_cases.json, written by
-tests/golden/make_fixtures.py with exact rational arithmetic): the oracle
-must reproduce every case bit for bit (CPU), and so must the HIP operators
-(GPU; SpMV within rtol = atol = 1e-6, since it sums products in double)."""
+tests/golden/make_fixtures.py with exact rational arithmetic and brute-force
+loops): the oracle must reproduce every case bit for bit (CPU), and so must
+the HIP operators (GPU; SpMV within rtol = atol = 1e-6, since it sums
+products in double).  Covers the scans, segmented sums, SpMV, gather,
+compaction and the consumers of scan (filters, count_if, split, sort,
+top-k, searchsorted, the segmented-scan revert)."""
 import json
 import os
 
@@ -170,3 +173,70 @@ def test_gpu_blocks(ops, dev):
                                 _t(np.array(c["sums_f32"], dtype=np.float32), dev), c["tile_width"],
                                 c["tile_height"])
     np.testing.assert_allclose(to_np(got), f32(c["y_f32"]), rtol=2e-6, atol=2e-6)
+
+
+# ---------------------------------------------------------------------------
+# Consumers of scan: oracle (CPU) and HIP operators (GPU) against fixtures
+# ---------------------------------------------------------------------------
+
+def test_oracle_consumers():
+    c = CASES["filters_f16"]
+    x = f16(c["x_f16"])
+    assert_exact(oracle.filter_cmp(x, c["pivot"], oracle.CMP_GE), f16(c["ge_f16"]))
+    assert_exact(oracle.filter_cmp(x, c["pivot"], oracle.CMP_LE), f16(c["le_f16"]))
+    assert_exact(oracle.filter_cmp(x, c["pivot"], oracle.CMP_GE, positions=True), np.array(c["where"], np.int32))
+    assert [oracle.count_if(x, c["pivot"], m) for m in range(6)] == c["count_if"]
+    c = CASES["split_i16"]
+    z, zi = oracle.split(i16(c["x_i16"]), np.array(c["mask"], np.int8), np.arange(1, len(c["mask"]) + 1))
+    assert_exact(z, i16(c["z_i16"]))
+    assert zi.tolist() == c["idx_out"]
+    c = CASES["sort_f16"]
+    assert oracle.sort(f16(c["x_f16"]))[1].tolist() == c["asc_idx"]
+    assert oracle.sort(f16(c["x_f16"]), descending=True)[1].tolist() == c["desc_idx"]
+    c = CASES["sort_i16"]
+    assert oracle.sort(i16(c["x_i16"]))[1].tolist() == c["asc_idx"]
+    assert oracle.sort(i16(c["x_i16"]), descending=True)[1].tolist() == c["desc_idx"]
+    c = CASES["topk_f16"]
+    assert oracle.topk(f16(c["x_f16"]), c["k"])[1].tolist() == c["largest_idx"]
+    assert oracle.topk(f16(c["x_f16"]), c["k"], largest=False)[1].tolist() == c["smallest_idx"]
+    c = CASES["searchsorted_i32"]
+    assert oracle.searchsorted(np.array(c["sorted"], np.int32), np.array(c["values"], np.int32)).tolist() == c["out"]
+    c = CASES["seg_scan_revert_f32"]
+    got = oracle.seg_scan_revert(np.array(c["scan_x"], np.float32), np.array(c["scan_f"]), np.array(c["seg_ends"]))
+    assert_exact(got, np.array(c["z"], np.float32))
+
+
+@pytest.mark.gpu
+def test_gpu_consumers(ops, dev):
+    c = CASES["filters_f16"]
+    x = _t(f16(c["x_f16"]), dev)
+    assert_exact(to_np(ops.run_filter_greater_eq(x, c["pivot"], 32)), f16(c["ge_f16"]))
+    assert_exact(to_np(ops.run_filter_less_eq(x, c["pivot"], 32)), f16(c["le_f16"]))
+    assert to_np(ops.run_where(x, c["pivot"], 32)).tolist() == c["where"]
+    assert [int(ops.run_count_if(x, c["pivot"], 128, m).item()) for m in range(6)] == c["count_if"]
+    c = CASES["split_i16"]
+    n = len(c["mask"])
+    z, zi = ops.run_split_ind(_t(i16(c["x_i16"]), dev), _t(np.array(c["mask"], np.int8), dev),
+                              torch.arange(1, n + 1, dtype=torch.int32, device=dev), 32)
+    assert_exact(to_np(z), i16(c["z_i16"]))
+    assert to_np(zi).tolist() == c["idx_out"]
+    c = CASES["sort_f16"]
+    xs = _t(f16(c["x_f16"]), dev)
+    v, i = ops.run_radix_sort(xs, 32)
+    assert to_np(i).tolist() == c["asc_idx"]
+    assert_exact(to_np(v).view(np.uint16), f16(c["x_f16"])[c["asc_idx"]].view(np.uint16))
+    assert to_np(ops.run_radix_sort_ex(xs, True)[1]).tolist() == c["desc_idx"]
+    c = CASES["sort_i16"]
+    assert to_np(ops.run_radix_sort(_t(i16(c["x_i16"]), dev), 32)[1]).tolist() == c["asc_idx"]
+    c = CASES["topk_f16"]
+    v, i = ops.run_topk(_t(f16(c["x_f16"]), dev), c["k"])
+    assert to_np(i).tolist() == c["largest_idx"]
+    assert to_np(ops.run_topk(_t(f16(c["x_f16"]), dev), c["k"], False)[1]).tolist() == c["smallest_idx"]
+    c = CASES["searchsorted_i32"]
+    got = ops.run_searchsorted(_t(np.array(c["sorted"], np.int32), dev), _t(np.array(c["values"], np.int32), dev))
+    assert to_np(got).tolist() == c["out"]
+    c = CASES["seg_scan_revert_f32"]
+    got = ops.run_seg_scan_mc_revert(_t(np.array(c["scan_x"], np.float32), dev),
+                                     _t(np.array(c["scan_f"], np.int32), dev),
+                                     _t(np.array(c["seg_ends"], np.float32), dev))
+    assert_exact(to_np(got), np.array(c["z"], np.float32))
