@@ -9,12 +9,12 @@
 // Two launches.  The row pass gives every row one thread: empty rows get
 // y[r] = 0 and every tile records the row holding its first element.  The
 // nnz pass walks the nonzeros in tiles of NW * J * 64 * G elements with the
-// same lane layout as the scans: a tile stages the row ends that fall in it
-// in LDS, each lane locates its first row end by binary search and merges
-// the ends with its elements to find row heads and row ends, a segmented
-// scan runs over the tile, and the carry of a row that began in an earlier
-// tile arrives through the decoupled look-back.  Row sums are stored at the
-// row ends.  The work per tile is proportional to its nonzeros, however the
+// same lane layout as the scans: the threads of a tile mark the row
+// boundaries that fall in it in two LDS bitmaps (row heads, row ends) and
+// record which row ends at each end position, so every lane reads its
+// group's heads and ends with one LDS load; a segmented scan runs over the
+// tile, and the carry of a row that began in an earlier tile arrives
+// through the decoupled look-back.  Row sums are stored at the row ends.  The work per tile is proportional to its nonzeros, however the
 // rows are distributed (lognormal rows of up to 20000 nonzeros are split
 // across tiles like any other).
 //
@@ -28,7 +28,6 @@ namespace tcs {
 
 constexpr int kSegNW = 8, kSegJ = 2, kSegG = 8;
 constexpr int64_t kSegTile = (int64_t)kSegNW * kSegJ * kWave * kSegG;  // 8192
-constexpr int kEndsCap = 4096;  // row ends staged in LDS per tile
 
 template <class IdxT>
 __device__ __forceinline__ int64_t row_end(const IdxT* __restrict__ ends, int64_t r, int64_t n_ends,
@@ -118,16 +117,6 @@ struct SrcSpmv {
   }
 };
 
-// First index k in [0, cnt) with list[k] >= key (cnt if none).
-__device__ __forceinline__ int lower_bound_u32(const uint32_t* list, int cnt, uint32_t key) {
-  int lo = 0, hi = cnt;
-  while (lo < hi) {
-    const int mid = (lo + hi) >> 1;
-    if (list[mid] < key) lo = mid + 1; else hi = mid;
-  }
-  return lo;
-}
-
 template <class Src, class IdxT, class OutT>
 __global__ __launch_bounds__(kSegNW * kWave) void seg_reduce_kernel(
     Src src, const IdxT* __restrict__ ends, int64_t R, int64_t n_ends, int64_t nnz,
@@ -135,7 +124,9 @@ __global__ __launch_bounds__(kSegNW * kWave) void seg_reduce_kernel(
   using Acc = typename Src::Acc;
   constexpr int G = kSegG, J = kSegJ, NW = kSegNW;
   constexpr int64_t GSPAN = (int64_t)kWave * G, WSEG = GSPAN * J;
-  __shared__ uint32_t s_list[kEndsCap];
+  __shared__ uint32_t s_head[kSegTile / 32];  // bit q: a row starts at t0 + q
+  __shared__ uint32_t s_endb[kSegTile / 32];  // bit q: a row ends at t0 + q (inclusive)
+  __shared__ uint32_t s_rowk[kSegTile];       // at an end position: r - (r_lo - 1) of that row
   __shared__ Acc s_wval[NW], s_wexcl[NW], s_carry;
   __shared__ int s_wflag[NW], s_wexclf[NW];
   __shared__ uint32_t s_tag;
@@ -150,36 +141,14 @@ __global__ __launch_bounds__(kSegNW * kWave) void seg_reduce_kernel(
   if (t0 >= end_all) return;  // no row reaches this tile (nor any later one)
   const int64_t t1 = t0 + kSegTile < end_all ? t0 + kSegTile : end_all;
 
-  // Rows touching the tile and the row ends around it: list[k] =
-  // end(r_lo - 1 + k).
+  // Rows touching the tile: r_lo (holds t0) .. r_hi; their ends are
+  // end(r_lo - 1 + k) for k in [0, cnt).
   int64_t r_lo = (int64_t)tile_row[tile];
   int64_t r_hi = tile + 1 < num_tiles ? (int64_t)tile_row[tile + 1] : R - 1;
   if (r_lo > R - 1) r_lo = R - 1;
   if (r_hi > R - 1) r_hi = R - 1;
   if (r_hi < r_lo) r_hi = r_lo;
-  const int64_t cnt64 = r_hi - r_lo + 2;
-  const bool in_lds = cnt64 <= kEndsCap;
-  const int cnt = in_lds ? (int)cnt64 : 0;
-  if (in_lds) {
-    for (int k = threadIdx.x; k < cnt; k += NW * kWave)
-      s_list[k] = (uint32_t)row_end(ends, r_lo - 1 + k, n_ends, nnz);
-  }
-  __syncthreads();
-
-  // Locates heads and ends in one lane group.  Returns the list index of
-  // the group's first relevant end, for the re-walk in the output pass.
-  auto list_at = [&](int64_t k) -> uint32_t {
-    return in_lds ? s_list[k] : (uint32_t)row_end(ends, r_lo - 1 + k, n_ends, nnz);
-  };
-  auto first_k = [&](int64_t gb) -> int64_t {
-    if (in_lds) return lower_bound_u32(s_list, cnt, (uint32_t)gb);
-    int64_t lo = 0, hi = cnt64;
-    while (lo < hi) {
-      const int64_t mid = (lo + hi) >> 1;
-      if (list_at(mid) < (uint64_t)gb) lo = mid + 1; else hi = mid;
-    }
-    return lo;
-  };
+  const int64_t cnt = r_hi - r_lo + 2;
 
   typename Src::Regs regs;
   const int64_t wbase = t0 + (int64_t)wid * WSEG + (int64_t)lane * G;
@@ -192,28 +161,37 @@ __global__ __launch_bounds__(kSegNW * kWave) void seg_reduce_kernel(
     for (int j = 0; j < J; ++j) src.template load<false>(regs, j, wbase + j * GSPAN, t1);
   }
 
-  int64_t kfirst[J];
+  // Mark the boundaries.  Among equal ends (empty rows) only the first, the
+  // row that holds the element before it, marks them, so every position is
+  // written by at most one thread.
+  for (int k = threadIdx.x; k < kSegTile / 32; k += NW * kWave) {
+    s_head[k] = 0;
+    s_endb[k] = 0;
+  }
+  __syncthreads();
+  for (int64_t k = threadIdx.x; k < cnt; k += NW * kWave) {
+    const int64_t e = row_end(ends, r_lo - 1 + k, n_ends, nnz);
+    const int64_t ep = k > 0 ? row_end(ends, r_lo - 2 + k, n_ends, nnz) : -1;
+    if (k > 0 && ep == e) continue;  // an empty row: the same boundary as before
+    if (e >= t0 && e < t1) {
+      const int q = (int)(e - t0);
+      atomicOr(&s_head[q >> 5], 1u << (q & 31));
+    }
+    if (e > t0 && e <= t1) {
+      const int q = (int)(e - 1 - t0);
+      atomicOr(&s_endb[q >> 5], 1u << (q & 31));
+      s_rowk[q] = (uint32_t)k;
+    }
+  }
+  __syncthreads();
+
+  // Heads and ends of each lane group (G = 8 elements at an 8-aligned offset).
   uint32_t hmask[J], emask[J];
 #pragma unroll
   for (int j = 0; j < J; ++j) {
-    const int64_t gb = wbase + j * GSPAN;
-    int64_t k = first_k(gb);
-    kfirst[j] = k;
-    uint32_t h = 0, e = 0;
-    for (int i = 0; i < G; ++i) {
-      const int64_t p = gb + i;
-      if (p >= t1) break;
-      // A row ending exactly at p means the next non-empty row starts at p.
-      bool head = false;
-      while (k < cnt64 && list_at(k) <= (uint64_t)p) {
-        head |= list_at(k) == (uint64_t)p;
-        ++k;
-      }
-      if (head || p == 0) h |= 1u << i;
-      if (k < cnt64 && list_at(k) == (uint64_t)(p + 1)) e |= 1u << i;
-    }
-    hmask[j] = h;
-    emask[j] = e;
+    const int q = (int)(wbase + j * GSPAN - t0);
+    hmask[j] = (s_head[q >> 5] >> (q & 31)) & 0xffu;
+    emask[j] = (s_endb[q >> 5] >> (q & 31)) & 0xffu;
   }
 
   // Segmented reduction of the tile (as in the scans).
@@ -309,11 +287,9 @@ __global__ __launch_bounds__(kSegNW * kWave) void seg_reduce_kernel(
   for (int j = 0; j < J; ++j) {
     if (!emask[j]) continue;
     Acc run = gexf[j] ? gex[j] : cin + gex[j];
-    int64_t k = kfirst[j];
-    const int64_t gb = wbase + j * GSPAN;
+    const int q0 = (int)(wbase + j * GSPAN - t0);
+#pragma unroll
     for (int i = 0; i < G; ++i) {
-      const int64_t p = gb + i;
-      if (p >= t1) break;
       const Acc v = src.value(regs, j, i);
       if (hmask[j] >> i & 1u) {
         run = v;
@@ -321,8 +297,7 @@ __global__ __launch_bounds__(kSegNW * kWave) void seg_reduce_kernel(
         run = run + v;
       }
       if (emask[j] >> i & 1u) {
-        while (k + 1 < cnt64 && list_at(k) != (uint64_t)(p + 1)) ++k;
-        int64_t row = r_lo - 1 + k;
+        int64_t row = r_lo - 1 + (int64_t)s_rowk[q0 + i];
         if (row < 0) row = 0;
         if (row > R - 1) row = R - 1;
         y[row] = (OutT)run;
