@@ -279,6 +279,21 @@ def test_searchsorted_edges_and_indptr(ops, dev):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("data_len", [16383, 16384, 16385, 32769, 1_000_003])
+def test_searchsorted_sample_windows(ops, dev, data_len):
+    # The kernel brackets each needle between two of every B-th haystack
+    # element staged in LDS, B = ceil(len / 16384): runs of equal values that
+    # straddle those positions, needles equal to, just below and just above
+    # haystack values, and more needles than one sweep of the grid covers.
+    r = np.random.default_rng(data_len)
+    a = np.sort(r.integers(-3000, 3000, data_len) * 7).astype(np.int32)
+    picks = a[r.integers(0, data_len, 1_500_000)]
+    v = (picks + r.integers(-1, 2, picks.size)).astype(np.int32)
+    v[:3] = [a[0] - 1, a[-1], a[-1] + 1]
+    assert_exact(_search(ops, dev, a, v), oracle.searchsorted(a, v))
+
+
+@pytest.mark.gpu
 def test_searchsorted_at_scale(ops, dev):
     r = np.random.default_rng(8)
     a = np.sort(r.integers(-2**31, 2**31 - 1, (1 << 24) + 5)).astype(np.int32)
