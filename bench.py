@@ -11,7 +11,11 @@ Printed: one JSON line with the driver's fields plus
                 stream it runs on, as achieved GB/s against 8 TB/s;
   cpu_baseline  the oracle's C port of the reference's run_scan_cpu
                 (three-phase OpenMP blocked scan), timed on this host over
-                whole copies of the same workload;
+                whole copies of the same workload; beside it the host (core
+                count, thread settings, lscpu model) and host libraries on
+                bounded samples: torch.cumsum, torch.masked_select (values
+                and positions), np.add.reduceat seg-sums and scipy's CSR
+                product on the SpMV matrix;
   extras        the other workloads measured the same way: the N = 2^20
                 fp16 scan's per-call latency (run_scan_single_core), the int8 and
                 bf16 scans, run_compress_ind on N = 2^28 fp32 values with a
@@ -470,6 +474,97 @@ def _cpu_baseline(n: int, budget_s: float = 10.0):
     }
 
 
+def _host_info():
+    """This host as the CPU baselines see it: core count, thread settings
+    and the lscpu model line."""
+    import subprocess
+
+    import torch
+
+    info = {"os_cpu_count": os.cpu_count(), "omp_num_threads": os.environ.get("OMP_NUM_THREADS"),
+            "torch_threads": torch.get_num_threads()}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in ("Model name", "CPU(s)", "Thread(s) per core", "Socket(s)"):
+                info["lscpu_" + k.strip().lower().replace(" ", "_").replace("(s)", "s")] = v.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return info
+
+
+def _time_cpu(fn, budget_s: float, max_reps: int = 50):
+    fn()  # warm (page-in, thread pool start)
+    reps, t0 = 0, time.perf_counter()
+    while True:
+        fn()
+        reps += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s or reps >= max_reps:
+            return el / reps, reps
+
+
+def _cpu_library_baselines(budget_s: float = 2.0):
+    """The host libraries beside the GPU ops, each on a bounded sample of
+    its workload: torch.cumsum for the scan, torch.masked_select for
+    run_compress_ind, np.add.reduceat for an indptr seg-sum and scipy's CSR
+    product (single-threaded) for the SpMV matrix itself."""
+    import numpy as np
+    import torch
+
+    out = {}
+    threads = torch.get_num_threads()
+    g = torch.Generator().manual_seed(0)
+
+    n = 1 << 26
+    x = (torch.rand(n, generator=g) * 2 - 1).half()
+    per, reps = _time_cpu(lambda: torch.cumsum(x, 0, dtype=torch.float32), budget_s)
+    out["torch_cumsum_f16_to_f32"] = {"value": round(6 * n / per / 1e9, 3), "unit": "GB/s", "cores": threads,
+                                      "sample": f"N=2^26, {reps} runs, {per * 1e3:.1f} ms each"}
+    del x
+
+    xf = torch.rand(n, generator=g)
+    m = torch.rand(n, generator=g) < 0.3
+    idx = torch.arange(n, dtype=torch.int32)
+    k = int(m.sum())
+    per, reps = _time_cpu(lambda: (torch.masked_select(xf, m), torch.masked_select(idx, m)), budget_s)
+    out["torch_masked_select_compress_ind_f32"] = {
+        "value": round((9 * n + 8 * k) / per / 1e9, 3), "unit": "GB/s", "cores": threads,
+        "sample": f"N=2^26 fp32, Bernoulli(0.3) mask, values and positions, {reps} runs, {per * 1e3:.1f} ms each"}
+    del xf, m, idx
+
+    rng = np.random.default_rng(0)
+    lens = rng.geometric(1 / 64, size=1 << 20)
+    starts = np.concatenate([[0], np.cumsum(lens)[:-1]]).astype(np.int64)
+    nnz = int(lens.sum())
+    xs = rng.integers(-4, 5, nnz).astype(np.int8)
+    per, reps = _time_cpu(lambda: np.add.reduceat(xs, starts, dtype=np.int32), budget_s)
+    out["numpy_reduceat_seg_sum_i8"] = {
+        "value": round((nnz + 8 * lens.size + 4) / per / 1e9, 3), "unit": "GB/s", "cores": 1,
+        "sample": f"{lens.size} segments of geometric length (mean 64), {nnz} int8 values, {reps} runs, "
+                  f"{per * 1e3:.1f} ms each"}
+    del xs, starts
+
+    try:
+        import scipy.sparse as sp
+    except ImportError:
+        sp = None
+    if sp is not None and torch.cuda.is_available():
+        vals, indptr, cols, xv, nnz = spmv_matrix(1 << 22)
+        rows = indptr.numel() - 1
+        A = sp.csr_matrix((vals.float().cpu().numpy(), cols.cpu().numpy(), indptr.cpu().numpy()),
+                          shape=(rows, rows))
+        xh = xv.float().cpu().numpy()
+        del vals, indptr, cols, xv
+        torch.cuda.empty_cache()
+        per, reps = _time_cpu(lambda: A @ xh, budget_s, max_reps=20)
+        out["scipy_csr_spmv_f32"] = {"value": round(2 * nnz / per / 1e9, 3), "unit": "GFLOP/s", "cores": 1,
+                                     "sample": f"the 2^22 x 2^22 lognormal matrix ({nnz} nonzeros, values as "
+                                               f"float32), {reps} runs, {per * 1e3:.1f} ms each"}
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -591,7 +686,11 @@ def main() -> int:
             "gbs": round(sv["gbs"], 2), "requests": sv["requests"], "seconds": sv["seconds"],
             "batches": sv["batches"], "per_kind": sv["per_kind"],
         }
-    cpu = None if a.no_cpu else _cpu_baseline(n)
+    cpu = None
+    if not a.no_cpu:
+        cpu = _cpu_baseline(n)
+        cpu["host"] = _host_info()
+        cpu["libraries"] = _cpu_library_baselines()
 
     ms_per_step = main_res["wall_s"] * 1e3 / a.steps
     out = {

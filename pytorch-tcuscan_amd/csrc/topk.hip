@@ -7,11 +7,12 @@
 // top: a histogram of digit r over the keys whose higher digits equal the
 // digits chosen so far, then one workgroup picks the bin holding the k-th
 // largest.  State passes between launches in the workspace, so no result
-// reaches the host until the end.  One ordered compaction (look-back on
-// packed (greater, equal) counts) then takes every key above T and the
-// first `need` keys equal to T in input order, and the stable radix sort
-// orders those k, largest first (ties by input position).  Reads: one pass
-// per key byte plus one.
+// reaches the host until the end.  An ordered selection then takes every
+// key above T and the first `need` keys equal to T in input order (per-tile
+// counts, their scan, and placement by the tiles that hold any of them),
+// and the stable radix sort orders those k, largest first (ties by input
+// position).  Reads: one pass per key byte plus one, plus the tiles that
+// hold selected keys.
 //
 // tcs_topk_pivot follows run_topk_pivot_fp16 (src/torch/torch_topk.h:
 // 23-60, src/kernels/kernel_topk_pivot.h): per block, over its first eight
@@ -26,6 +27,9 @@ namespace tcs {
 
 constexpr int kTkThreads = 512, kTkMaxGroups = 1024, kTkPickThreads = 1024;
 constexpr int kTkNW = 8, kTkG = 8;
+// Placement walks the tiles with a fixed grid: most tiles place nothing
+// when k is small, and a workgroup per tile would cost a launch each.
+constexpr uint32_t kTkPlaceGrid = 2048;
 
 template <int DT>
 struct TopkGeom {
@@ -59,29 +63,34 @@ __global__ __launch_bounds__(kTkThreads) void topk_hist_kernel(const typename So
   const uint32_t want = hs >= KT::kBits ? 0u : (uint32_t)(state[0] >> hs);
   const int64_t c0 = (int64_t)blockIdx.x * chunk;
   const int64_t c1 = c0 + chunk < n ? c0 + chunk : n;
-  // Four 16-byte loads in flight per thread per step.
+  // Four 16-byte loads in flight per thread per step; steps that lie wholly
+  // inside the chunk skip the per-element bounds checks.
   constexpr int U = 4;
   constexpr int64_t STEP = (int64_t)kTkThreads * V;
-  for (int64_t b0 = c0 + (int64_t)tid * V; b0 < c1; b0 += U * STEP) {
+  auto count = [&](const Raw (&v)[V]) {
+#pragma unroll
+    for (int i = 0; i < V; ++i) {
+      const uint32_t k = KT::key(v[i]) ^ flip;
+      if (hs >= KT::kBits || (k >> hs) == want) atomicAdd(&h[hc][(k >> (8 * r)) & 0xffu], 1u);
+    }
+  };
+  int64_t b0 = c0 + (int64_t)tid * V;
+  const int64_t full_end = c0 + ((c1 - c0) / (U * STEP)) * (U * STEP);  // steps [c0, full_end) are whole
+  for (; b0 < full_end; b0 += U * STEP) {
     Raw v[U][V];
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int64_t b = b0 + u * STEP;
-      if (b + V <= c1) {
-        load_items<Raw, V, true>(x, b, c1, v[u]);
-      } else {
-        load_items<Raw, V, false>(x, b, c1, v[u]);
-      }
-    }
+    for (int u = 0; u < U; ++u) load_items<Raw, V, true>(x, b0 + u * STEP, c1, v[u]);
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int64_t b = b0 + u * STEP;
+    for (int u = 0; u < U; ++u) count(v[u]);
+  }
+  for (; b0 < c1; b0 += STEP) {
+    Raw v[V];
+    load_items<Raw, V, false>(x, b0, c1, v);
 #pragma unroll
-      for (int i = 0; i < V; ++i) {
-        const uint32_t k = KT::key(v[u][i]) ^ flip;
-        const bool match = b + i < c1 && (hs >= KT::kBits || (k >> hs) == want);
-        if (match) atomicAdd(&h[hc][(k >> (8 * r)) & 0xffu], 1u);
-      }
+    for (int i = 0; i < V; ++i) {
+      const uint32_t k = KT::key(v[i]) ^ flip;
+      const bool match = b0 + i < c1 && (hs >= KT::kBits || (k >> hs) == want);
+      if (match) atomicAdd(&h[hc][(k >> (8 * r)) & 0xffu], 1u);
     }
   }
   __syncthreads();
@@ -91,6 +100,28 @@ __global__ __launch_bounds__(kTkThreads) void topk_hist_kernel(const typename So
     for (int w = 0; w < NH; ++w) s += h[w][e];
     partial[(size_t)blockIdx.x * 256 + e] = s;
   }
+}
+
+// First level of the digit-count sum: workgroup q adds the partials of
+// groups [q * per, (q + 1) * per) into psum[q] (64-bit, high half zero), so
+// that the pick reads kTkSumGroups rows instead of one per histogram group.
+constexpr int kTkSumGroups = 32;
+__global__ __launch_bounds__(256) void topk_sum_kernel(const uint64_t* __restrict__ partial, int groups, int per,
+                                                       uint64_t* __restrict__ psum) {
+  const int d = threadIdx.x;
+  const int g0 = blockIdx.x * per;
+  const int g1 = g0 + per < groups ? g0 + per : groups;
+  uint32_t s = 0;
+  int g = g0;
+  for (; g + 8 <= g1; g += 8) {  // eight loads in flight
+    uint64_t v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = partial[(size_t)(g + u) * 256 + d];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) s += (uint32_t)v[u];
+  }
+  for (; g < g1; ++g) s += (uint32_t)partial[(size_t)g * 256 + d];
+  psum[(size_t)blockIdx.x * 256 + d] = s;
 }
 
 // Picks digit r of the k-th largest key: the bin b where the count of keys
@@ -141,107 +172,180 @@ __global__ __launch_bounds__(kTkPickThreads) void topk_pick_kernel(const uint64_
 
 // ---------------------------------------------------------------------------
 // Ordered selection: keys above T go to [0, k - need) in input order, the
-// first `need` keys equal to T to [k - need, k).
+// first `need` keys equal to T to [k - need, k).  Three launches with no
+// look-back: per-tile counts, one scan of them, then placement, in which a
+// tile that places nothing (most of them when k is small) exits after
+// reading two words.
 // ---------------------------------------------------------------------------
-template <int DT>
-__global__ __launch_bounds__(kTkNW * kWave) void topk_select_kernel(
-    const typename SortKey<DT>::Raw* __restrict__ x, int64_t n, uint32_t flip,
-    const uint64_t* __restrict__ state, int64_t k, typename SortKey<DT>::Raw* __restrict__ cand_v,
-    int32_t* __restrict__ cand_i, void* ws, uint32_t num_tiles) {
+
+// Loads tile `tile` and sets, per lane group, the bits of the keys above T
+// (gtb) and equal to T (eqb); FULL: the tile lies wholly inside [0, n).
+// A tile holds at most 16384 keys, so a count
+// packed as greater << 16 | equal never carries between the halves.
+template <int DT, bool FULL>
+__device__ __forceinline__ void topk_classify(const typename SortKey<DT>::Raw* __restrict__ x, int64_t n,
+                                              uint32_t flip, uint32_t T, uint32_t tile, int wid, int lane,
+                                              Packed<typename SortKey<DT>::Raw, kTkG> (&xv)[TopkGeom<DT>::J],
+                                              uint32_t (&gtb)[TopkGeom<DT>::J], uint32_t (&eqb)[TopkGeom<DT>::J]) {
   using KT = SortKey<DT>;
   using Raw = typename KT::Raw;
-  constexpr int G = kTkG, J = TopkGeom<DT>::J, NW = kTkNW;
+  constexpr int G = kTkG, J = TopkGeom<DT>::J;
   constexpr int64_t TILE = TopkGeom<DT>::kTile;
+  static_assert(TILE <= 16384, "packed 16-bit tile counts");
   constexpr int64_t GSPAN = (int64_t)kWave * G, WSEG = GSPAN * J;
-  __shared__ int64_t s_wval[NW], s_wexcl[NW];
-  __shared__ int64_t s_base;
-  __shared__ uint32_t s_tag;
-  if (threadIdx.x == 0) s_tag = launch_arrive(ws);
-  const uint32_t T = (uint32_t)state[0];
-  const int64_t need = (int64_t)state[1];
-  const int64_t c_gt = k - need;
-  const int lane = __lane_id(), wid = threadIdx.x / kWave;
-  const uint32_t tile = blockIdx.x;
-  const int64_t t0 = (int64_t)tile * TILE;
-  const int64_t wbase = t0 + (int64_t)wid * WSEG + (int64_t)lane * G;
-
-  // Packed counts: greater << 32 | equal (each below 2^31).
-  auto packed_of = [&](Raw v, bool valid) -> int64_t {
-    const uint32_t kk = KT::key(v) ^ flip;
-    return !valid ? 0 : (kk > T ? ((int64_t)1 << 32) : (kk == T ? 1 : 0));
-  };
-  uint32_t gtb[J], eqb[J];
-  Packed<Raw, G> xv[J];
-  int64_t gex[J];
-  int64_t wtot = 0;
+  const int64_t wbase = (int64_t)tile * TILE + (int64_t)wid * WSEG + (int64_t)lane * G;
 #pragma unroll
   for (int j = 0; j < J; ++j) {
     const int64_t gb = wbase + j * GSPAN;
-    if (t0 + TILE <= n) {
-      load_items<Raw, G, true>(x, gb, n, xv[j]);
-    } else {
-      load_items<Raw, G, false>(x, gb, n, xv[j]);
-    }
+    load_items<Raw, G, FULL>(x, gb, n, xv[j]);
     uint32_t g = 0, e = 0;
 #pragma unroll
     for (int i = 0; i < G; ++i) {
       const uint32_t kk = KT::key(xv[j][i]) ^ flip;
-      const bool valid = gb + i < n;
+      const bool valid = FULL || gb + i < n;
       g |= (valid && kk > T ? 1u : 0u) << i;
       e |= (valid && kk == T ? 1u : 0u) << i;
     }
     gtb[j] = g;
     eqb[j] = e;
-    const int64_t c = ((int64_t)__builtin_popcount(g) << 32) | __builtin_popcount(e);
-    const int64_t incl = wave_incl_scan(c);
-    gex[j] = wtot + incl - c;
-    wtot += readlane(incl, kWave - 1);
   }
-  if (lane == 0) s_wval[wid] = wtot;
+}
+
+template <int DT>
+__device__ __forceinline__ void topk_classify_any(const typename SortKey<DT>::Raw* __restrict__ x, int64_t n,
+                                                  uint32_t flip, uint32_t T, uint32_t tile, int wid, int lane,
+                                                  Packed<typename SortKey<DT>::Raw, kTkG> (&xv)[TopkGeom<DT>::J],
+                                                  uint32_t (&gtb)[TopkGeom<DT>::J],
+                                                  uint32_t (&eqb)[TopkGeom<DT>::J]) {
+  if ((int64_t)(tile + 1) * TopkGeom<DT>::kTile <= n) {
+    topk_classify<DT, true>(x, n, flip, T, tile, wid, lane, xv, gtb, eqb);
+  } else {
+    topk_classify<DT, false>(x, n, flip, T, tile, wid, lane, xv, gtb, eqb);
+  }
+}
+
+// tcnt[tile] = (keys above T) << 32 | (keys equal to T) in the tile.
+template <int DT>
+__global__ __launch_bounds__(kTkNW * kWave) void topk_tilecount_kernel(
+    const typename SortKey<DT>::Raw* __restrict__ x, int64_t n, uint32_t flip,
+    const uint64_t* __restrict__ state, uint64_t* __restrict__ tcnt) {
+  constexpr int J = TopkGeom<DT>::J, NW = kTkNW;
+  __shared__ int s_w[NW];
+  const int lane = __lane_id(), wid = threadIdx.x / kWave;
+  Packed<typename SortKey<DT>::Raw, kTkG> xv[J];
+  uint32_t gtb[J], eqb[J];
+  topk_classify_any<DT>(x, n, flip, (uint32_t)state[0], blockIdx.x, wid, lane, xv, gtb, eqb);
+  int c = 0;
+#pragma unroll
+  for (int j = 0; j < J; ++j) c += (__builtin_popcount(gtb[j]) << 16) | __builtin_popcount(eqb[j]);
+  c = wave_sum(c);
+  if (lane == 0) s_w[wid] = c;
   __syncthreads();
-  if (wid == 0) {
-    const int64_t a = lane < NW ? s_wval[lane] : 0;
-    const int64_t incl = wave_incl_scan(a);
-    const int64_t agg = readlane(incl, NW - 1);
-    if (lane < NW) s_wexcl[lane] = incl - a;
-    auto fb = [&](int64_t u) {
-      const int64_t u0 = u * TILE;
-      const int64_t u1 = u0 + TILE < n ? u0 + TILE : n;
-      int64_t c = 0;
-      for (int64_t p = u0 + lane; p < u1; p += kWave) c += packed_of(x[p], true);
-      return FlagVal<int64_t>{u == 0 ? 1 : 0, wave_sum(c)};
-    };
-    uint64_t* desc = ws_desc(ws);
-    const uint32_t tag = s_tag;
-    int64_t carry = 0;
-    if (tile == 0) {
-      if (lane == 0 && num_tiles > 1) desc_publish<int64_t>(desc, 0, agg, tag, kFlagIncl);
-    } else {
-      if (lane == 0) desc_publish<int64_t>(desc, tile, agg, tag, kFlagAgg);
-      carry = lookback<int64_t>(desc, tile, tag, fb);
-      if (lane == 0) desc_publish<int64_t>(desc, tile, carry + agg, tag, kFlagIncl);
+  if (threadIdx.x == 0) {
+    int t = 0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) t += s_w[w];
+    tcnt[blockIdx.x] = ((uint64_t)((uint32_t)t >> 16) << 32) | ((uint32_t)t & 0xffffu);
+  }
+}
+
+// In place: tcnt[t] = sum of tcnt over the tiles before t, for t in
+// [0, tiles]; one workgroup, 16 consecutive tiles per thread per round.
+// The packed halves sum without a carry (each total is below 2^31).
+constexpr int kTkScanThreads = 1024, kTkScanPer = 16;
+__global__ __launch_bounds__(kTkScanThreads) void topk_tilescan_kernel(uint64_t* __restrict__ tcnt,
+                                                                       uint32_t tiles) {
+  constexpr int NWV = kTkScanThreads / kWave;
+  constexpr int64_t ROUND = (int64_t)kTkScanThreads * kTkScanPer;
+  __shared__ int64_t s_w[NWV];
+  const int tid = threadIdx.x, lane = __lane_id(), wid = tid / kWave;
+  int64_t carry = 0;
+  for (int64_t r0 = 0; r0 < tiles; r0 += ROUND) {
+    const int64_t t0 = r0 + (int64_t)tid * kTkScanPer;
+    int64_t v[kTkScanPer];
+    int64_t sum = 0;
+#pragma unroll
+    for (int q = 0; q < kTkScanPer; ++q) {
+      v[q] = t0 + q < tiles ? (int64_t)tcnt[t0 + q] : 0;
+      sum += v[q];
     }
-    if (lane == 0) s_base = carry;
+    const int64_t incl = wave_incl_scan(sum);
+    if (lane == kWave - 1) s_w[wid] = incl;
+    __syncthreads();
+    int64_t before = carry, total = carry;
+#pragma unroll
+    for (int w = 0; w < NWV; ++w) {
+      const int64_t sw = s_w[w];
+      before += w < wid ? sw : 0;
+      total += sw;
+    }
+    before += incl - sum;
+#pragma unroll
+    for (int q = 0; q < kTkScanPer; ++q) {
+      if (t0 + q < tiles) tcnt[t0 + q] = (uint64_t)before;
+      before += v[q];
+    }
+    carry = total;
+    __syncthreads();  // s_w is rewritten by the next round
   }
-  __syncthreads();
-  const int64_t base = s_base + s_wexcl[wid];
+  if (tid == 0) tcnt[tiles] = (uint64_t)carry;
+}
+
+template <int DT>
+__global__ __launch_bounds__(kTkNW * kWave) void topk_place_kernel(
+    const typename SortKey<DT>::Raw* __restrict__ x, int64_t n, uint32_t flip,
+    const uint64_t* __restrict__ state, int64_t k, const uint64_t* __restrict__ tcnt,
+    typename SortKey<DT>::Raw* __restrict__ cand_v, int32_t* __restrict__ cand_i, uint32_t num_tiles) {
+  constexpr int G = kTkG, J = TopkGeom<DT>::J, NW = kTkNW;
+  constexpr int64_t TILE = TopkGeom<DT>::kTile;
+  constexpr int64_t GSPAN = (int64_t)kWave * G, WSEG = GSPAN * J;
+  __shared__ int s_wval[NW];
+  const int64_t need = (int64_t)state[1];
+  const int64_t c_gt = k - need;
+  const int lane = __lane_id(), wid = threadIdx.x / kWave;
+  for (uint32_t tile = blockIdx.x; tile < num_tiles; tile += gridDim.x) {
+    const uint64_t e0 = tcnt[tile], e1 = tcnt[tile + 1];
+    const int64_t gt0 = (int64_t)(e0 >> 32), eq0 = (int64_t)(e0 & 0xffffffffu);
+    const int64_t gth = (int64_t)(e1 >> 32) - gt0, eqh = (int64_t)(e1 & 0xffffffffu) - eq0;
+    if (gth == 0 && (eqh == 0 || eq0 >= need)) continue;  // the same for the whole workgroup
+
+    Packed<typename SortKey<DT>::Raw, G> xv[J];
+    uint32_t gtb[J], eqb[J];
+    topk_classify_any<DT>(x, n, flip, (uint32_t)state[0], tile, wid, lane, xv, gtb, eqb);
+    int gex[J];
+    int wtot = 0;
 #pragma unroll
-  for (int j = 0; j < J; ++j) {
-    const int64_t before = base + gex[j];
-    int64_t gpos = before >> 32, epos = before & 0xffffffffll;
-    const int64_t gb = wbase + j * GSPAN;
+    for (int j = 0; j < J; ++j) {
+      const int c = (__builtin_popcount(gtb[j]) << 16) | __builtin_popcount(eqb[j]);
+      const int incl = wave_incl_scan(c);
+      gex[j] = wtot + incl - c;
+      wtot += readlane(incl, kWave - 1);
+    }
+    if (lane == 0) s_wval[wid] = wtot;
+    __syncthreads();
+    int wex = 0;
 #pragma unroll
-    for (int i = 0; i < G; ++i) {
-      if ((gtb[j] >> i) & 1u) {
-        cand_v[gpos] = xv[j][i];
-        cand_i[gpos] = (int32_t)(gb + i);
-        ++gpos;
-      } else if ((eqb[j] >> i) & 1u) {
-        if (epos < need) {
-          cand_v[c_gt + epos] = xv[j][i];
-          cand_i[c_gt + epos] = (int32_t)(gb + i);
+    for (int w = 0; w < NW; ++w) wex += w < wid ? s_wval[w] : 0;
+    __syncthreads();  // s_wval is rewritten for the next tile
+    const int64_t wbase = (int64_t)tile * TILE + (int64_t)wid * WSEG + (int64_t)lane * G;
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+      const int before = wex + gex[j];
+      int64_t gpos = gt0 + (before >> 16), epos = eq0 + (before & 0xffff);
+      const int64_t gb = wbase + j * GSPAN;
+#pragma unroll
+      for (int i = 0; i < G; ++i) {
+        if ((gtb[j] >> i) & 1u) {
+          cand_v[gpos] = xv[j][i];
+          cand_i[gpos] = (int32_t)(gb + i);
+          ++gpos;
+        } else if ((eqb[j] >> i) & 1u) {
+          if (epos < need) {
+            cand_v[c_gt + epos] = xv[j][i];
+            cand_i[c_gt + epos] = (int32_t)(gb + i);
+          }
+          ++epos;
         }
-        ++epos;
       }
     }
   }
@@ -323,8 +427,8 @@ inline int topk_elem_bytes(int dt) {
 }
 
 struct TopkLayout {
-  size_t state_off, part_off, own_total;
-  size_t cand_i_off, sort_scratch_off, scratch;
+  size_t state_off, part_off, psum_off, own_total;
+  size_t cand_i_off, tcnt_off, sort_scratch_off, scratch;
   int groups;
   int64_t chunk;
 };
@@ -341,9 +445,11 @@ inline TopkLayout topk_layout(int64_t n, int64_t k, int dt) {
   L.chunk = chunk;
   L.state_off = lookback_ws_bytes(tiles);
   L.part_off = align256(L.state_off + 256);
-  L.own_total = align256(L.part_off + (size_t)groups * 256 * 8);
+  L.psum_off = align256(L.part_off + (size_t)groups * 256 * 8);
+  L.own_total = align256(L.psum_off + (size_t)kTkSumGroups * 256 * 8);
   L.cand_i_off = align256((size_t)k * topk_elem_bytes(dt));
-  L.sort_scratch_off = align256(L.cand_i_off + (size_t)k * 4);
+  L.tcnt_off = align256(L.cand_i_off + (size_t)k * 4);
+  L.sort_scratch_off = align256(L.tcnt_off + (size_t)(tiles + 1) * 8);
   L.scratch = L.sort_scratch_off + tcs_sort_scratch_bytes(k, dt);
   return L;
 }
@@ -358,6 +464,9 @@ static int topk_run(const void* x, int64_t n, int64_t k, bool largest, void* out
   char* w = static_cast<char*>(ws);
   auto* state = reinterpret_cast<uint64_t*>(w + L.state_off);
   auto* partial = reinterpret_cast<uint64_t*>(w + L.part_off);
+  auto* psum = reinterpret_cast<uint64_t*>(w + L.psum_off);
+  const int sum_per = (L.groups + kTkSumGroups - 1) / kTkSumGroups;
+  const int sum_groups = (L.groups + sum_per - 1) / sum_per;
   char* sc = static_cast<char*>(scratch);
   auto* cand_v = reinterpret_cast<Raw*>(sc);
   auto* cand_i = reinterpret_cast<int32_t*>(sc + L.cand_i_off);
@@ -367,14 +476,22 @@ static int topk_run(const void* x, int64_t n, int64_t k, bool largest, void* out
     hipLaunchKernelGGL((topk_hist_kernel<DT>), dim3(L.groups), dim3(kTkThreads), 0, st, xr, n, L.chunk, flip, r,
                        state, partial);
     TCS_CHECK_LAUNCH();
-    hipLaunchKernelGGL(topk_pick_kernel, dim3(1), dim3(kTkPickThreads), 0, st, partial, L.groups, r,
+    hipLaunchKernelGGL(topk_sum_kernel, dim3(sum_groups), dim3(256), 0, st, partial, L.groups, sum_per, psum);
+    TCS_CHECK_LAUNCH();
+    hipLaunchKernelGGL(topk_pick_kernel, dim3(1), dim3(kTkPickThreads), 0, st, psum, sum_groups, r,
                        r == P - 1 ? 1 : 0, k, state);
     TCS_CHECK_LAUNCH();
   }
   constexpr int64_t kTile = TopkGeom<DT>::kTile;
   const uint32_t tiles = (uint32_t)((n + kTile - 1) / kTile);
-  hipLaunchKernelGGL((topk_select_kernel<DT>), dim3(tiles), dim3(kTkNW * kWave), 0, st, xr, n, flip, state, k,
-                     cand_v, cand_i, ws, tiles);
+  auto* tcnt = reinterpret_cast<uint64_t*>(sc + L.tcnt_off);
+  hipLaunchKernelGGL((topk_tilecount_kernel<DT>), dim3(tiles), dim3(kTkNW * kWave), 0, st, xr, n, flip, state, tcnt);
+  TCS_CHECK_LAUNCH();
+  hipLaunchKernelGGL(topk_tilescan_kernel, dim3(1), dim3(kTkScanThreads), 0, st, tcnt, tiles);
+  TCS_CHECK_LAUNCH();
+  const uint32_t place_grid = tiles < kTkPlaceGrid ? tiles : kTkPlaceGrid;
+  hipLaunchKernelGGL((topk_place_kernel<DT>), dim3(place_grid), dim3(kTkNW * kWave), 0, st, xr, n, flip, state, k,
+                     tcnt, cand_v, cand_i, tiles);
   TCS_CHECK_LAUNCH();
   // Order the k selected: largest first (smallest first for !largest), ties
   // in input order.  The sort reuses this workspace (same header and epoch).
