@@ -55,31 +55,33 @@ __global__ __launch_bounds__(kHistThreads) void sort_hist_kernel(const typename 
   __syncthreads();
   const int64_t c0 = (int64_t)blockIdx.x * chunk;
   const int64_t c1 = c0 + chunk < n ? c0 + chunk : n;
-  // Four 16-byte loads in flight per thread per step.
+  // Four 16-byte loads in flight per thread per step; steps that lie wholly
+  // inside the chunk skip the per-element bounds checks.
   constexpr int U = 4;
   constexpr int64_t STEP = (int64_t)kHistThreads * V;
-  for (int64_t b0 = c0 + (int64_t)tid * V; b0 < c1; b0 += U * STEP) {
+  auto add = [&](Raw r) {
+    const uint32_t k = KT::key(r) ^ flip;
+#pragma unroll
+    for (int p = 0; p < P; ++p) atomicAdd(&h[hc][p * 256 + ((k >> (8 * p)) & 0xffu)], 1u);
+  };
+  int64_t b0 = c0 + (int64_t)tid * V;
+  const int64_t full_end = c0 + ((c1 - c0) / (U * STEP)) * (U * STEP);  // steps [c0, full_end) are whole
+  for (; b0 < full_end; b0 += U * STEP) {
     Raw r[U][V];
 #pragma unroll
+    for (int u = 0; u < U; ++u) load_items<Raw, V, true>(x, b0 + u * STEP, c1, r[u]);
+#pragma unroll
     for (int u = 0; u < U; ++u) {
-      const int64_t b = b0 + u * STEP;
-      if (b + V <= c1) {
-        load_items<Raw, V, true>(x, b, c1, r[u]);
-      } else {
-        load_items<Raw, V, false>(x, b, c1, r[u]);
-      }
+#pragma unroll
+      for (int i = 0; i < V; ++i) add(r[u][i]);
     }
+  }
+  for (; b0 < c1; b0 += STEP) {
+    Raw r[V];
+    load_items<Raw, V, false>(x, b0, c1, r);
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int64_t b = b0 + u * STEP;
-#pragma unroll
-      for (int i = 0; i < V; ++i) {
-        if (b + i < c1) {
-          const uint32_t k = KT::key(r[u][i]) ^ flip;
-#pragma unroll
-          for (int p = 0; p < P; ++p) atomicAdd(&h[hc][p * 256 + ((k >> (8 * p)) & 0xffu)], 1u);
-        }
-      }
+    for (int i = 0; i < V; ++i) {
+      if (b0 + i < c1) add(r[i]);
     }
   }
   __syncthreads();
@@ -89,6 +91,28 @@ __global__ __launch_bounds__(kHistThreads) void sort_hist_kernel(const typename 
     for (int w = 0; w < NH; ++w) s += h[w][e];
     partial[(size_t)blockIdx.x * P * 256 + e] = s;  // 64-bit: high half zero
   }
+}
+
+// First level of the digit-count sum: workgroup (q, p) adds pass p's
+// partials of groups [q * per, (q + 1) * per) into psum (same layout as the
+// partials, kSortSumGroups rows; 64-bit words, high half zero).
+constexpr int kSortSumGroups = 32;
+__global__ __launch_bounds__(256) void sort_sum_kernel(const uint64_t* __restrict__ partial, int groups, int per,
+                                                       int passes, uint64_t* __restrict__ psum) {
+  const int d = threadIdx.x, q = blockIdx.x, p = blockIdx.y;
+  const int g0 = q * per;
+  const int g1 = g0 + per < groups ? g0 + per : groups;
+  uint32_t s = 0;
+  int g = g0;
+  for (; g + 8 <= g1; g += 8) {  // eight loads in flight
+    uint64_t v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = partial[((size_t)(g + u) * passes + p) * 256 + d];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) s += (uint32_t)v[u];
+  }
+  for (; g < g1; ++g) s += (uint32_t)partial[((size_t)g * passes + p) * 256 + d];
+  psum[((size_t)q * passes + p) * 256 + d] = s;
 }
 
 // gbase[p * 256 + d] = number of keys whose digit p is below d.  One
@@ -147,6 +171,10 @@ __global__ __launch_bounds__(kSortNW * kWave) void sort_pass_kernel(
   __shared__ uint32_t s_tag;
   __shared__ Raw s_keys[kSortTile];
   __shared__ int32_t s_vals[kSortTile];
+  // Digits beside the reordered keys for keys of up to 2 bytes; 4-byte keys
+  // recompute them (the extra 8 KiB would leave one workgroup per CU).
+  constexpr bool kDigLds = sizeof(Raw) <= 2;
+  __shared__ uint8_t s_dig[kDigLds ? kSortTile : 1];
 
   const int tid = threadIdx.x, lane = __lane_id(), wid = tid / kWave;
   if (tid == 0) s_tag = launch_arrive(ws);
@@ -156,42 +184,53 @@ __global__ __launch_bounds__(kSortNW * kWave) void sort_pass_kernel(
   const int64_t wb = t0 + (int64_t)wid * (kWave * IT);
   auto digit_of = [&](Raw r) { return (int)(((KT::key(r) ^ flip) >> shift) & 0xffu); };
 
+  // A tile wholly inside [0, n) takes the unchecked path (every tile but
+  // the last).
+  const bool full = t0 + kSortTile <= n;
   Raw kr[IT];
   int32_t vv[IT];
-#pragma unroll
-  for (int i = 0; i < IT; ++i) {
-    const int64_t p = wb + i * kWave + lane;
-    const bool valid = p < n;
-    kr[i] = valid ? kin[p] : Raw(0);
-    if constexpr (GEN_IDX) {
-      vv[i] = (int32_t)p;
-    } else {
-      vv[i] = valid ? vin[p] : 0;
-    }
-  }
-  __syncthreads();
-
-  // Stable rank of every item among the wave's items of the same digit.
-  const uint64_t lt = (1ull << lane) - 1ull;
   uint32_t dp[IT];  // digit << 16 | position within the wave's digit bucket
+  auto load_rank = [&](auto full_c) {
+    constexpr bool FULL = decltype(full_c)::value;
 #pragma unroll
-  for (int i = 0; i < IT; ++i) {
-    const bool valid = wb + i * kWave + lane < n;
-    const int d = digit_of(kr[i]);
-    uint64_t peers = __ballot(valid);
+    for (int i = 0; i < IT; ++i) {
+      const int64_t p = wb + i * kWave + lane;
+      const bool valid = FULL || p < n;
+      kr[i] = valid ? kin[p] : Raw(0);
+      if constexpr (GEN_IDX) {
+        vv[i] = (int32_t)p;
+      } else {
+        vv[i] = valid ? vin[p] : 0;
+      }
+    }
+    __syncthreads();
+
+    // Stable rank of every item among the wave's items of the same digit.
+    const uint64_t lt = (1ull << lane) - 1ull;
 #pragma unroll
-    for (int b = 0; b < 8; ++b) {
-      const bool bit = (d >> b) & 1;
-      const uint64_t m = __ballot(bit);
-      peers &= bit ? m : ~m;
+    for (int i = 0; i < IT; ++i) {
+      const bool valid = FULL || wb + i * kWave + lane < n;
+      const int d = digit_of(kr[i]);
+      uint64_t peers = FULL ? ~0ull : __ballot(valid);
+#pragma unroll
+      for (int b = 0; b < 8; ++b) {
+        const bool bit = (d >> b) & 1;
+        const uint64_t m = __ballot(bit);
+        peers &= bit ? m : ~m;
+      }
+      const uint32_t rank = (uint32_t)__popcll(peers & lt);
+      uint32_t base = 0;
+      if (valid) {
+        base = s_whist[wid][d];
+        if (rank == 0) s_whist[wid][d] = base + (uint32_t)__popcll(peers);
+      }
+      dp[i] = ((uint32_t)d << 16) | (base + rank);
     }
-    const uint32_t rank = (uint32_t)__popcll(peers & lt);
-    uint32_t base = 0;
-    if (valid) {
-      base = s_whist[wid][d];
-      if (rank == 0) s_whist[wid][d] = base + (uint32_t)__popcll(peers);
-    }
-    dp[i] = ((uint32_t)d << 16) | (base + rank);
+  };
+  if (full) {
+    load_rank(std::integral_constant<bool, true>{});
+  } else {
+    load_rank(std::integral_constant<bool, false>{});
   }
   __syncthreads();
 
@@ -253,30 +292,33 @@ __global__ __launch_bounds__(kSortNW * kWave) void sort_pass_kernel(
   // Reorder the tile by digit in LDS (stable), then write each digit's run.
 #pragma unroll
   for (int i = 0; i < IT; ++i) {
-    if (wb + i * kWave + lane < n) {
+    if (full || wb + i * kWave + lane < n) {
       const int d = (int)(dp[i] >> 16);
       const uint32_t pos = s_excl[d] + s_whist[wid][d] + (dp[i] & 0xffffu);
       s_keys[pos] = kr[i];
       s_vals[pos] = vv[i];
+      if constexpr (kDigLds) s_dig[pos] = (uint8_t)d;
     }
   }
   __syncthreads();
-  const int tvalid = n - t0 < kSortTile ? (int)(n - t0) : (int)kSortTile;
+  const int tvalid = full ? (int)kSortTile : (int)(n - t0);
   for (int pos = tid; pos < tvalid; pos += NW * kWave) {
     const Raw r = s_keys[pos];
-    const int64_t dst = (int64_t)s_gofs[digit_of(r)] + pos;
+    const int d = kDigLds ? (int)s_dig[pos] : digit_of(r);
+    const int64_t dst = (int64_t)s_gofs[d] + pos;
     kout[dst] = r;
     vout[dst] = s_vals[pos];
   }
 }
 
 // Workspace: header, 256 descriptors per tile, bucket starts, histogram
-// partials (zeroed once and reusable, like every workspace; counts are
-// kept in 64-bit words with a zero high half, which never read as a valid
-// descriptor when a later call uses the same bytes for descriptors).
+// partials and their first-level sums (zeroed once and reusable, like every
+// workspace; counts are kept in 64-bit words with a zero high half, which
+// never read as a valid descriptor when a later call uses the same bytes
+// for descriptors).
 // Scratch: one temporary copy of keys and positions (any contents).
 struct SortLayout {
-  size_t desc_off, gbase_off, part_off, total;
+  size_t desc_off, gbase_off, part_off, psum_off, total;
   size_t tmpv_off, scratch;
   int groups;
   int64_t chunk;
@@ -309,7 +351,8 @@ inline SortLayout sort_layout(int64_t n, int dt) {
   L.desc_off = kHeaderBytes;
   L.gbase_off = align256(L.desc_off + (size_t)tiles * 256 * 8);
   L.part_off = align256(L.gbase_off + (size_t)kMaxPasses * 256 * 8);
-  L.total = align256(L.part_off + (size_t)groups * P * 256 * 8);
+  L.psum_off = align256(L.part_off + (size_t)groups * P * 256 * 8);
+  L.total = align256(L.psum_off + (size_t)kSortSumGroups * P * 256 * 8);
   L.tmpv_off = align256((size_t)n * sort_elem_bytes(dt));
   L.scratch = align256(L.tmpv_off + (size_t)n * 4);
   return L;
@@ -331,7 +374,12 @@ static int sort_run(const void* x, const int32_t* idx_in, void* out_x, int32_t* 
   hipLaunchKernelGGL((sort_hist_kernel<DT>), dim3(L.groups), dim3(kHistThreads), 0, st, static_cast<const Raw*>(x),
                      n, L.chunk, flip, partial);
   TCS_CHECK_LAUNCH();
-  hipLaunchKernelGGL(sort_bucket_kernel, dim3(P), dim3(kBucketThreads), 0, st, partial, L.groups, P, gbase);
+  auto* psum = reinterpret_cast<uint64_t*>(w + L.psum_off);
+  const int sum_per = (L.groups + kSortSumGroups - 1) / kSortSumGroups;
+  const int sum_groups = (L.groups + sum_per - 1) / sum_per;
+  hipLaunchKernelGGL(sort_sum_kernel, dim3(sum_groups, P), dim3(256), 0, st, partial, L.groups, sum_per, P, psum);
+  TCS_CHECK_LAUNCH();
+  hipLaunchKernelGGL(sort_bucket_kernel, dim3(P), dim3(kBucketThreads), 0, st, psum, sum_groups, P, gbase);
   TCS_CHECK_LAUNCH();
   const uint32_t tiles = (uint32_t)((n + kSortTile - 1) / kSortTile);
   const Raw* kin = static_cast<const Raw*>(x);
