@@ -498,33 +498,51 @@ __global__ __launch_bounds__(kCountBlock) void count_partials_kernel(const XT* _
                                                                      int64_t* __restrict__ partial) {
   const int64_t c0 = (int64_t)blockIdx.x * kCountChunk;
   const int64_t c1 = c0 + kCountChunk < n ? c0 + kCountChunk : n;
-  int c = 0;
-  const bool full = c1 - c0 == kCountChunk;
-  constexpr int V = Keep::kFromX ? (int)(16 / sizeof(XT)) : 16;  // items per 16-byte load
-  for (int64_t b = c0 + (int64_t)threadIdx.x * V; b < c1; b += kCountBlock * V) {
+  // Each wave counts its kept items with ballots (one compare per item, the
+  // popcount on the scalar unit); a full chunk keeps four 16-byte loads in
+  // flight per thread and needs no bounds checks.
+  using LT = typename std::conditional<Keep::kFromX, XT, int8_t>::type;
+  constexpr int V = (int)(16 / sizeof(LT));  // items per 16-byte load
+  constexpr int U = 4;
+  constexpr int64_t STEP = (int64_t)kCountBlock * V;
+  static_assert(kCountChunk % (U * STEP) == 0, "full chunks are whole unrolled steps");
+  const LT* src;
+  if constexpr (Keep::kFromX) {
+    src = x;
+  } else {
+    src = keep.mask;
+  }
+  auto kept = [&](LT v) -> bool {
     if constexpr (Keep::kFromX) {
-      XT v[V];
-      if (full) {
-        load_items<XT, V, true>(x, b, c1, v);
-      } else {
-        load_items<XT, V, false>(x, b, c1, v);
-      }
-#pragma unroll
-      for (int i = 0; i < V; ++i) c += (full || b + i < c1) && keep.of(v[i]) ? 1 : 0;
+      return keep.of(v);
     } else {
-      int8_t m[16];
-      if (full) {
-        load_items<int8_t, 16, true>(keep.mask, b, c1, m);
-      } else {
-        load_items<int8_t, 16, false>(keep.mask, b, c1, m);
-      }
+      return v == 1;
+    }
+  };
+  uint32_t c = 0;  // the same in every lane of a wave
+  int64_t b = c0 + (int64_t)threadIdx.x * V;
+  if (c1 - c0 == kCountChunk) {
+    for (; b < c1; b += U * STEP) {
+      LT v[U][V];
 #pragma unroll
-      for (int i = 0; i < 16; ++i) c += m[i] == 1;
+      for (int u = 0; u < U; ++u) load_items<LT, V, true>(src, b + u * STEP, c1, v[u]);
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+#pragma unroll
+        for (int i = 0; i < V; ++i) c += (uint32_t)__popcll(__ballot(kept(v[u][i])));
+      }
+    }
+  } else {
+    // The trip count is the same for every lane (lanes past c1 load zeros).
+    for (int64_t b0 = c0; b0 < c1; b0 += STEP, b += STEP) {
+      LT v[V];
+      load_items<LT, V, false>(src, b, c1, v);
+#pragma unroll
+      for (int i = 0; i < V; ++i) c += (uint32_t)__popcll(__ballot(b + i < c1 && kept(v[i])));
     }
   }
-  __shared__ int s[kCountBlock / kWave];
-  const int w = wave_sum(c);
-  if (__lane_id() == 0) s[threadIdx.x / kWave] = w;
+  __shared__ uint32_t s[kCountBlock / kWave];
+  if (__lane_id() == 0) s[threadIdx.x / kWave] = c;
   __syncthreads();
   if (threadIdx.x == 0) {
     int64_t t = 0;
@@ -535,14 +553,23 @@ __global__ __launch_bounds__(kCountBlock) void count_partials_kernel(const XT* _
 
 __global__ __launch_bounds__(kCountBlock) void count_final_kernel(const int64_t* __restrict__ partial,
                                                                   int64_t np, int64_t* __restrict__ out) {
+  // Four partials in flight per thread; fixed summation order.
   int64_t t = 0;
-  for (int64_t k = threadIdx.x; k < np; k += kCountBlock) t += partial[k];
-  __shared__ int64_t s[kCountBlock];
-  s[threadIdx.x] = t;
+  int64_t k = threadIdx.x;
+  for (; k + 3 * kCountBlock < np; k += 4 * kCountBlock) {
+    const int64_t a = partial[k], b = partial[k + kCountBlock], c = partial[k + 2 * kCountBlock],
+                  d = partial[k + 3 * kCountBlock];
+    t += (a + b) + (c + d);
+  }
+  for (; k < np; k += kCountBlock) t += partial[k];
+  __shared__ int64_t s[kCountBlock / kWave];
+  const int64_t w = wave_sum(t);
+  if (__lane_id() == 0) s[threadIdx.x / kWave] = w;
   __syncthreads();
   if (threadIdx.x == 0) {
     int64_t a = 0;
-    for (int k = 0; k < kCountBlock; ++k) a += s[k];
+#pragma unroll
+    for (int q = 0; q < kCountBlock / kWave; ++q) a += s[q];
     out[0] = a;
   }
 }

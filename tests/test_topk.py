@@ -155,6 +155,35 @@ def test_topk_at_scale(ops, dev):
 
 
 @pytest.mark.gpu
+def test_topk_more_tiles_than_one_scan_round(ops, dev):
+    # 2^28 + 17 int8 keys: 16385 tiles of 16384, so the tile-count scan runs
+    # two rounds, with heavy ties at the k-th value (256 distinct keys).
+    # Expected: every key beyond the k-th value T, then the first `need`
+    # positions holding T, ordered by value (ties by position).
+    n = (1 << 28) + 17
+    g = torch.Generator(device=dev).manual_seed(11)
+    xd = torch.randint(-128, 128, (n,), generator=g, device=dev, dtype=torch.int16).to(torch.int8)
+    x = xd.cpu().numpy()
+    cnt = np.bincount(x.astype(np.int16) + 128, minlength=256)
+    for k, largest in ((1000, True), (3_000_000, False)):
+        v, i = ops.run_topk(xd, k, largest)
+        acc = 0
+        for b in (range(255, -1, -1) if largest else range(256)):
+            if acc + cnt[b] >= k:
+                T = b - 128
+                break
+            acc += cnt[b]
+        beyond = np.flatnonzero(x > T) if largest else np.flatnonzero(x < T)
+        need = k - beyond.size
+        sel = np.concatenate([beyond, np.flatnonzero(x == T)[:need]])
+        key = -x[sel].astype(np.int16) if largest else x[sel].astype(np.int16)
+        ei = sel[np.lexsort((sel, key))].astype(np.int32)
+        assert_exact(to_np(i), ei, f"k={k} largest={largest}")
+        assert_exact(to_np(v), x[ei], f"k={k} largest={largest} values")
+    del xd
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("num_samples", [4, 16, 32, 64])
 def test_topk_pivot_matches_oracle(ops, dev, num_samples):
     for n in (1000, 100_000, 3_000_017):
