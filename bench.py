@@ -22,7 +22,9 @@ Printed: one JSON line with the driver's fields plus
                 Bernoulli(0.3) mask, run_filter_greater_eq on N = 2^28 fp16
                 normals (pivot 0.5), run_radix_sort and run_topk_fp16
                 (k = 2048) on N = 2^28 fp16 normals with torch.sort /
-                torch.topk timed beside them, run_searchsorted of 2^24 int32 needles
+                torch.topk timed beside them, tcuscan.sample_top_p on 2^24
+                fp16 probabilities (p = 0.1) beside the same recipe on
+                torch.sort and torch.cumsum, run_searchsorted of 2^24 int32 needles
                 into 2^24 sorted values, and run_spmv_v2 on the 2^22 x 2^22 CSR
                 matrix with lognormal row lengths (mean 16, sigma 1.2,
                 clipped to [0, 20000]; about 6.7e7 nonzeros).
@@ -224,6 +226,35 @@ def _sort_workload(n: int, dtype_name: str, steps: int, warmup: int):
     torch.cuda.empty_cache()
     return {"wall_s": wall, "event_avg_ms": sum(ev_ms) / len(ev_ms), "gkeys_per_s": n * steps / wall / 1e9,
             "torch_sort_ms": twall * 1e3 / tsteps, "torch_gkeys_per_s": n * tsteps / twall / 1e9}
+
+
+def _top_p_workload(n: int, steps: int, warmup: int, p: float = 0.1):
+    """tcuscan.sample_top_p on N fp16 probabilities uniform in [0, 1)
+    (p = 0.1, one draw), the reference's top-p benchmark input, with the same
+    recipe on torch.sort(descending) and torch.cumsum timed beside it."""
+    import torch
+
+    import tcuscan
+
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device=dev).manual_seed(0)
+    probs = torch.rand(n, generator=g, device=dev).half()
+
+    def torch_top_p():
+        srt, idx = torch.sort(probs, descending=True)
+        mass = torch.cumsum(srt, 0, dtype=torch.float32)
+        kept = srt.float()
+        kept = torch.where(mass - kept > p, torch.zeros_like(kept), kept)
+        kept = kept / kept.sum()
+        return idx.gather(0, torch.multinomial(kept, 1))
+
+    torch.cuda.synchronize()
+    wall, ev_ms = _timed_steps(lambda: tcuscan.sample_top_p(probs, p), steps, warmup)
+    twall, _ = _timed_steps(torch_top_p, steps, warmup, spin_s=0.1)
+    del probs
+    torch.cuda.empty_cache()
+    return {"wall_s": wall, "ms_per_call": wall * 1e3 / steps, "event_avg_ms": sum(ev_ms) / len(ev_ms),
+            "torch_top_p_ms": twall * 1e3 / steps}
 
 
 def _topk_workload(n: int, k: int, steps: int, warmup: int):
@@ -580,7 +611,7 @@ def main() -> int:
     ap.add_argument("--only", choices=["scan_f16", "scan_i8", "scan_bf16", "scan_f16_mfma", "scan_i8_mfma",
                                        "scan_f16_dpp", "scan_i8_dpp", "compress_ind", "spmv", "copy_f32",
                                        "batched", "serve", "filter_ge", "searchsorted", "sort_f16", "sort_i32",
-                                       "topk_f16"],
+                                       "topk_f16", "top_p"],
                     help="run one workload and print its timing only (for profilers)")
     a = ap.parse_args()
     if a.gpus != 1:
@@ -609,6 +640,8 @@ def main() -> int:
             r = _searchsorted_workload(24, 24, a.steps, a.warmup)
         elif a.only == "topk_f16":
             r = _topk_workload(n, 2048, a.steps, a.warmup)
+        elif a.only == "top_p":
+            r = _top_p_workload(1 << 24, a.steps, a.warmup)
         elif a.only in ("sort_f16", "sort_i32"):
             r = _sort_workload(n, "fp16" if a.only == "sort_f16" else "int32", a.steps, a.warmup)
         else:
@@ -659,6 +692,11 @@ def main() -> int:
         extras[f"topk_f16_2p{a.log2n}_k2048"] = {
             "value": round(tk["ms_per_call"], 4), "unit": "ms per call",
             "torch_topk_ms": round(tk["torch_topk_ms"], 4),
+        }
+        tp = _top_p_workload(1 << 24, ksteps, 2)
+        extras["sample_top_p_f16_2p24"] = {
+            "value": round(tp["ms_per_call"], 4), "unit": "ms per call",
+            "torch_top_p_ms": round(tp["torch_top_p_ms"], 4),
         }
         ss = _searchsorted_workload(24, 24, ksteps, max(3, a.warmup // 2))
         extras["searchsorted_i32_2p24_into_2p24"] = {

@@ -353,23 +353,35 @@ __global__ __launch_bounds__((kCmpNW + 1) * kWave) void filter_kernel(
   // Kept bits, per-group exclusive offsets inside the wave, wave total.
   auto count = [&](Regs& r, int (&gex)[J], CmpSlot<NW>& sl, uint32_t t) {
     const int64_t wbase = (int64_t)t * TILE + (int64_t)wid * WSEG + (int64_t)lane * G;
+    // Kept bits; a tile wholly inside [0, n) needs no bounds checks.
+    auto bits = [&](auto full_c) {
+      constexpr bool FULL = decltype(full_c)::value;
+#pragma unroll
+      for (int j = 0; j < J; ++j) {
+        const int64_t gb = wbase + j * GSPAN;
+        uint32_t b = 0;
+#pragma unroll
+        for (int i = 0; i < G; ++i) {
+          bool k;
+          if constexpr (kMask) {
+            k = r.mk[j][i] == 1;  // items past n load as 0
+          } else {
+            k = (FULL || gb + i < n) && keep.of(r.xv[j][i]);
+          }
+          b |= (k ? 1u : 0u) << i;
+        }
+        r.mb[j] = b;
+      }
+    };
+    if ((int64_t)(t + 1) * TILE <= n) {
+      bits(std::integral_constant<bool, true>{});
+    } else {
+      bits(std::integral_constant<bool, false>{});
+    }
     int wtot = 0;
 #pragma unroll
     for (int j = 0; j < J; ++j) {
-      const int64_t gb = wbase + j * GSPAN;
-      uint32_t b = 0;
-#pragma unroll
-      for (int i = 0; i < G; ++i) {
-        bool k;
-        if constexpr (kMask) {
-          k = r.mk[j][i] == 1;  // items past n load as 0
-        } else {
-          k = gb + i < n && keep.of(r.xv[j][i]);
-        }
-        b |= (k ? 1u : 0u) << i;
-      }
-      r.mb[j] = b;
-      const int c = __builtin_popcount(b);
+      const int c = __builtin_popcount(r.mb[j]);
       const int incl = wave_incl_scan(c);
       gex[j] = wtot + incl - c;
       wtot += readlane(incl, kWave - 1);
