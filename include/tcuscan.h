@@ -281,6 +281,14 @@ int tcs_filter(const void* x, int x_dtype, double pivot, int cmp, void* out_vals
 int tcs_seg_scan_revert(const float* scan_x, const int32_t* scan_f, const float* seg_ends, int64_t num_seg_ends,
                         float* z, int64_t n, void* stream);
 
+/* Top-p kept weights over float16 probabilities sorted largest first and
+ * their float32 inclusive prefix mass: kept[i] = (mass[i] - x[i] > p) ? 0 :
+ * x[i] in float32 arithmetic.  All three buffers 16-byte aligned.  Replaces
+ * the mask step `probs_sum - probs_sort > p; probs_sort[mask] = 0` of the
+ * reference's top-p recipe (scripts/profiling/profile_tcuscan_ops.py:
+ * 352-353 and 392-393), which is torch code there. */
+int tcs_top_p_keep(const void* sorted_probs, const float* mass, float p, float* kept, int64_t n, void* stream);
+
 /* ------------------------------------------------------------------------
  * Split (stable partition)
  * --------------------------------------------------------------------- */

@@ -555,6 +555,25 @@ at::Tensor run_seg_scan_mc_revert(const at::Tensor& scan_x_in, const at::Tensor&
   return z;
 }
 
+// Top-p mask step (scripts/profiling/profile_tcuscan_ops.py:352-353, torch
+// code in the reference): kept = where(mass - x > p, 0, x) in float32, for
+// the probabilities x sorted largest first and their inclusive mass.
+at::Tensor run_top_p_keep(const at::Tensor& sorted_in, const at::Tensor& mass_in, double p) {
+  const char* op = "run_top_p_keep";
+  require_device(sorted_in, op, "probs_sort");
+  require_device(mass_in, op, "probs_sum");
+  TORCH_CHECK(sorted_in.scalar_type() == at::kHalf, op, ": probs_sort must be float16, got ",
+              sorted_in.scalar_type());
+  TORCH_CHECK(mass_in.scalar_type() == at::kFloat, op, ": probs_sum must be float32, got ", mass_in.scalar_type());
+  TORCH_CHECK(sorted_in.numel() == mass_in.numel(), op, ": probs_sort and probs_sum must have the same length");
+  at::Tensor x = prep(sorted_in), m = prep(mass_in);
+  const at::hip::HIPGuardMasqueradingAsCUDA guard(x.device());
+  at::Tensor kept = at::empty({x.numel()}, m.options());
+  check(tcs_top_p_keep(x.data_ptr(), m.data_ptr<float>(), (float)p, kept.data_ptr<float>(), x.numel(), stream_of(x)),
+        op);
+  return kept;
+}
+
 // src/torch/torch_gather.h:92-172
 at::Tensor run_csr_gather(const at::Tensor& values_in, const at::Tensor& cols_in, const at::Tensor& x_in) {
   const char* op = "run_csr_gather";
@@ -1164,6 +1183,8 @@ PYBIND11_MODULE(tcuscan_ops, m) {
         py::arg("cols"), py::arg("x"), py::arg("upper"), py::arg("lower_strict"),
         "SpMV with the multi-cube block scan (fp16)");
   m.def("run_seg_scan_mc_revert", &tcuscan::run_seg_scan_mc_revert, "Vector Revert for MC Segmented Scan");
+  m.def("run_top_p_keep", &tcuscan::run_top_p_keep, "Top-p kept weights (mask step of nucleus sampling)",
+        py::arg("probs_sort"), py::arg("probs_sum"), py::arg("p"));
   m.def("run_compress", &tcuscan::run_compress, "Compaction/compress");
   m.def("run_compress_pos", &tcuscan::run_compress_pos,
         "Compaction/compress with pre-computed output positions");

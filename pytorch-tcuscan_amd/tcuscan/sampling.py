@@ -6,15 +6,28 @@ scan them, zero every entry whose preceding mass already exceeds p,
 renormalise, draw with torch.multinomial and map the draw back through the
 sort's indices.  Here the sort is run_radix_sort_ex (descending, stable: the
 order of sample_top_p at :372-392; the reference's tcuscan_top_p sorts
-ascending) and the scan run_scan_multi_core (fp16 in, float32 out, each
-prefix rounded once).  The mask, renormalisation and draw are elementwise
-torch ops on the device.
+ascending), the scan run_scan_multi_core (fp16 in, float32 out, each prefix
+rounded once) and the mask one fused kernel (run_top_p_keep).
+
+sample_top_p draws by inverting the kept weights' cumulative distribution:
+their inclusive scan (run_scan_single_core, float32 rounded once per prefix)
+and one uniform variate located by binary search.  That is the same
+distribution as renormalising and calling torch.multinomial, with no
+renormalisation pass, but not the same random stream.
 """
 from __future__ import annotations
 
 import torch
 
 import tcuscan_ops
+
+
+def _kept(probs: torch.Tensor, p: float, S: int):
+    if probs.dim() != 1 or probs.dtype != torch.float16:
+        raise RuntimeError("top_p: probs must be a 1-D float16 tensor")
+    probs_sort, order = tcuscan_ops.run_radix_sort_ex(probs, True)
+    mass = tcuscan_ops.run_scan_multi_core(probs_sort, S)
+    return tcuscan_ops.run_top_p_keep(probs_sort, mass, p), order
 
 
 def top_p_filter(probs: torch.Tensor, p: float, S: int = 128):
@@ -24,19 +37,18 @@ def top_p_filter(probs: torch.Tensor, p: float, S: int = 128):
 
     probs: 1-D float16 on the GPU.
     """
-    if probs.dim() != 1 or probs.dtype != torch.float16:
-        raise RuntimeError("top_p_filter: probs must be a 1-D float16 tensor")
-    probs_sort, order = tcuscan_ops.run_radix_sort_ex(probs, True)
-    mass = tcuscan_ops.run_scan_multi_core(probs_sort, S)
-    kept = probs_sort.float()
-    kept = torch.where(mass - kept > p, torch.zeros_like(kept), kept)
-    kept = kept / kept.sum()
-    return kept, order
+    kept, order = _kept(probs, p, S)
+    return kept / kept.sum(), order
 
 
 def sample_top_p(probs: torch.Tensor, p: float, S: int = 128, generator: torch.Generator | None = None):
     """One token index (int64 tensor of shape [1]) drawn from the top-p
     nucleus of `probs`."""
-    kept, order = top_p_filter(probs, p, S)
-    pick = torch.multinomial(kept, num_samples=1, generator=generator)
-    return order.long().gather(0, pick)
+    kept, order = _kept(probs, p, S)
+    cdf = tcuscan_ops.run_scan_single_core(kept, S, 0.0)
+    total = cdf[-1:]
+    u = torch.rand(1, generator=generator, device=probs.device) * total
+    pick = torch.searchsorted(cdf, u, right=True)
+    # u rounds up to the total at most: never past the last kept entry.
+    last = torch.searchsorted(cdf, total)
+    return order.gather(0, torch.minimum(pick, last)).long()

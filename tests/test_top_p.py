@@ -5,8 +5,10 @@ CPU: the oracle against the reference's own recipe written with torch's CPU
 operators (sort descending, cumsum, mask, renormalise;
 scripts/profiling/profile_tcuscan_ops.py:372-392).  GPU: the kept set and
 order exactly as the oracle's (the prefix sums are rounded once in both),
-the renormalised probabilities within 1e-6, and draws that only ever land
-in the nucleus.
+the renormalised probabilities within 1e-6, the fused mask kernel bit-exact
+against torch's evaluation of the reference's mask expression, and draws
+that only ever land in the nucleus, with frequencies matching the kept
+weights.
 """
 import numpy as np
 import pytest
@@ -58,3 +60,46 @@ def test_sample_top_p_draws_from_the_nucleus(dev):
     assert all(d in nucleus for d in draws)
     with pytest.raises(RuntimeError):
         tcuscan.top_p_filter(x.float(), 0.3)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n", [1, 7, 8, 1000, 65_537])
+def test_top_p_keep_is_torchs_mask_expression(ops, dev, n):
+    # kept = where(mass - x > p, 0, x) in float32, as torch evaluates the
+    # reference's `probs_sum - probs_sort > p` (bit-exact), also on views
+    # that start off the 16-byte grid.
+    x = _probs(n + 1, n).to(dev)
+    srt, _ = ops.run_radix_sort_ex(x, True)
+    mass = ops.run_scan_multi_core(srt, 128)
+    for p in (0.0, 0.3, 0.9, 2.0):
+        for a, m in ((srt, mass), (srt[1:], mass[1:])):
+            got = ops.run_top_p_keep(a, m, p)
+            ref = torch.where(m - a.float() > p, torch.zeros_like(m), a.float())
+            assert torch.equal(got, ref), (n, p)
+    with pytest.raises(RuntimeError):
+        ops.run_top_p_keep(srt.float(), mass, 0.1)
+    with pytest.raises(RuntimeError):
+        ops.run_top_p_keep(srt, mass[:-1], 0.1)
+
+
+@pytest.mark.gpu
+def test_sample_top_p_frequencies(dev):
+    # Draw frequencies of the inverse-CDF sampler against the renormalised
+    # kept weights (each within 5 standard deviations), and p = 0: the
+    # nucleus is the largest probability alone.
+    import tcuscan
+    w = torch.tensor([8, 5, 4, 3, 2, 1, 1, 0, 6], dtype=torch.float32)
+    x = (w / w.sum()).half().to(dev)
+    kept, order = tcuscan.top_p_filter(x, 0.7)
+    prob = {int(i): float(k) for i, k in zip(order.cpu(), kept.cpu()) if k > 0}
+    g = torch.Generator(device=dev).manual_seed(3)
+    draws = 3000
+    counts = {}
+    for _ in range(draws):
+        d = int(tcuscan.sample_top_p(x, 0.7, generator=g))
+        counts[d] = counts.get(d, 0) + 1
+    assert set(counts) <= set(prob)
+    for i, q in prob.items():
+        sd = (draws * q * (1 - q)) ** 0.5
+        assert abs(counts.get(i, 0) - draws * q) <= 5 * sd + 1, (i, counts.get(i, 0), draws * q)
+    assert all(int(tcuscan.sample_top_p(x, 0.0, generator=g)) == 0 for _ in range(20))
