@@ -92,7 +92,7 @@ enum tcs_scan_method {
 };
 
 /* tcs_scan with an explicit local scan method and look-back window
- * (predecessor tiles polled per step, 1..64; 0 = the default: 8 for DPP, 32
+ * (predecessor tiles polled per step, 1..64; 0 = the default: 6 for DPP, 32
  * for MFMA); results are
  * identical for every choice (bit-exact).  TCS_METHOD_MFMA multiplies each 1024-element
  * unit, as 32 rows of 32, by a lower-triangular ones matrix on
@@ -400,7 +400,7 @@ size_t tcs_batch_workspace_bytes(const tcs_batch* b);
 /* Autotuner.  For every scan op, times each candidate tiling -- tiles of
  * S*S*tiles_per_wg elements for S in {32, 64, 128} and tiles_per_wg in
  * {1, ..., 32} (distinct tiles only), the local scan method (DPP, and MFMA
- * for plain F16/I8 scans) and look-back windows {8, 16, 32} -- on the op's
+ * for plain F16/I8 scans) and look-back windows {6, 8, 16, 32} -- on the op's
  * own buffers with HIP events (one warm-up, then `reps` timed runs), and
  * keeps the fastest.  Results are cached per (op kind, dtype, ceil(log2 n))
  * for the process; `force` re-tunes cached keys.  Synchronises `stream`. */

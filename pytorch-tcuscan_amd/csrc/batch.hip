@@ -101,7 +101,7 @@ std::vector<ScanCfg> candidates(const Op& op) {
       if (std::find(seen.begin(), seen.end(), tile) != seen.end()) continue;
       seen.push_back(tile);
       for (int m : methods)
-        for (int win : {8, 16, 32}) {
+        for (int win : {6, 8, 16, 32}) {
           ScanCfg c;
           c.S = S;
           c.tiles_per_wg = tpw;

@@ -668,9 +668,10 @@ static int scan_dispatch(const void* x, int x_dtype, void* y, const int8_t* flag
   // N = 2^28 with the default tiles: DPP for every dtype (fp16 5.6 vs 3.7
   // TB/s on the matrix units; int8 5.1 vs 4.4).
   if (method == TCS_METHOD_AUTO) method = TCS_METHOD_DPP;
-  // Default look-back window, measured the same way: 8 for DPP (int8: 5.09
-  // vs 4.87 TB/s at 16), 32 for the matrix-unit scan.
-  if (window == 0) window = method == TCS_METHOD_MFMA ? 32 : 8;
+  // Default look-back window, measured the same way: 6 for DPP (fp16 265 us
+  // at 6 vs 272 at 8, 284 at 16, 420 at 1; bf16 263 vs 270; int8 level from
+  // 5 to 8), 32 for the matrix-unit scan.
+  if (window == 0) window = method == TCS_METHOD_MFMA ? 32 : 6;
   if (n == 0) return TCS_OK;
   if (!x || !y || (SEG == kSegFlags && !flags)) return TCS_ERR_INVALID_ARG;
   if (SEG == kSegFixed && seg_len <= 0) return TCS_ERR_INVALID_ARG;
