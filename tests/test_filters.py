@@ -294,6 +294,22 @@ def test_searchsorted_sample_windows(ops, dev, data_len):
 
 
 @pytest.mark.gpu
+def test_searchsorted_large_haystack_with_ties(ops, dev):
+    # 2^22 needles into a 2^24 haystack with ties (B = 1024-element windows
+    # below the LDS samples); every needle in one window; every needle below
+    # the haystack; against the oracle.
+    r = np.random.default_rng(21)
+    a = np.sort(r.integers(-10**6, 10**6, 1 << 24)).astype(np.int32)
+    v = np.concatenate([a[r.integers(0, a.size, 1 << 21)], r.integers(-10**6 - 5, 10**6 + 5, 1 << 21)])
+    v = v.astype(np.int32)
+    assert_exact(_search(ops, dev, a, v), oracle.searchsorted(a, v), "2^22 needles into 2^24")
+    one = (a[5_000_000] + r.integers(-1, 2, 1 << 20)).astype(np.int32)
+    assert_exact(_search(ops, dev, a, one), oracle.searchsorted(a, one), "one window")
+    low = np.full(1 << 17, a[0] - 3, dtype=np.int32)
+    assert _search(ops, dev, a, low).max() == 0
+
+
+@pytest.mark.gpu
 def test_searchsorted_at_scale(ops, dev):
     r = np.random.default_rng(8)
     a = np.sort(r.integers(-2**31, 2**31 - 1, (1 << 24) + 5)).astype(np.int32)
