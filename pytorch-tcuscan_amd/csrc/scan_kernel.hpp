@@ -687,10 +687,11 @@ static int scan_dispatch(const void* x, int x_dtype, void* y, const int8_t* flag
                  : (x_dtype == TCS_F32 || x_dtype == TCS_I32) ? 16384
                  : i8_plain_dpp ? 65536 : 32768;
   // Short inputs take smaller DPP tiles, so that more workgroups share
-  // them (fp16 at 2^20: 12.0 us per call at 8K vs 13.9 at 32K; 2^16: 8.8
-  // vs 11.1; at 2^22 16K and 32K tie).
+  // them (fp16 back to back, look-back window 6: 2^18 8.4 us at 8K, 8.6 at
+  // 16K; 2^20 10.4 at 8K, 10.0 at 16K, 13.9 at 32K; 2^21 12.1 at 8K, 11.2 at
+  // 16K; at 2^22 16K and 32K tie).
   if (tiles_per_wg <= 0 && method == TCS_METHOD_DPP) {
-    if (n <= ((int64_t)1 << 21) && tile > 8192) tile = 8192;
+    if (n <= ((int64_t)1 << 18) && tile > 8192) tile = 8192;
     else if (n <= ((int64_t)1 << 23) && tile > 16384) tile = 16384;
   }
   // Lane group width: 4 for 32-bit inputs, 8 for 16-bit and int8, 16 for
