@@ -91,11 +91,29 @@ struct SrcSpmv {
   const In* __restrict__ xv;
   int64_t x_len;
   struct Regs { In v[kSegJ][kSegG]; In g[kSegJ][kSegG]; };
+  // Values and column indices stream through once: nontemporal loads on
+  // whole groups, so that they do not displace the gathered vector x from
+  // the caches.
+  template <class T>
+  __device__ __forceinline__ static void load_nt(const T* __restrict__ p, int64_t base, T (&r)[kSegG]) {
+    constexpr int kBytes = kSegG * (int)sizeof(T);
+    static_assert(kBytes % 16 == 0, "whole 16-byte vectors");
+    const u32x4* vp = reinterpret_cast<const u32x4*>(p + base);
+    u32x4 tmp[kBytes / 16];
+#pragma unroll
+    for (int k = 0; k < kBytes / 16; ++k) tmp[k] = __builtin_nontemporal_load(vp + k);
+    __builtin_memcpy(r, tmp, kBytes);
+  }
   template <bool FULL>
   __device__ __forceinline__ void load(Regs& r, int j, int64_t gb, int64_t t1) const {
-    load_items<In, kSegG, FULL>(vals, gb, t1, r.v[j]);
     int32_t c[kSegG];
-    load_items<int32_t, kSegG, FULL>(cols, gb, t1, c);
+    if constexpr (FULL) {
+      load_nt<In>(vals, gb, r.v[j]);
+      load_nt<int32_t>(cols, gb, c);
+    } else {
+      load_items<In, kSegG, false>(vals, gb, t1, r.v[j]);
+      load_items<int32_t, kSegG, false>(cols, gb, t1, c);
+    }
 #pragma unroll
     for (int i = 0; i < kSegG; ++i)
       r.g[j][i] = ((uint32_t)c[i] < (uint64_t)x_len && (FULL || gb + i < t1)) ? xv[c[i]] : In(0);
