@@ -12,9 +12,14 @@
  * matching *_workspace_bytes function.
  *
  * Workspace contract (look-back kernels).  A workspace must be all-zero
- * before its first use.  Each call leaves it ready for the next call, so a
- * workspace is allocated and zeroed once and then reused, by one stream at
- * a time (calls that overlap on one workspace race).
+ * before its first use (tcs_workspace_reset zeroes one on a stream).  Each
+ * call leaves it ready for the next call, so a workspace is allocated and
+ * zeroed once and then reused, by one stream at a time (calls that overlap
+ * on one workspace race), and re-zeroed at least once every
+ * TCS_WORKSPACE_MAX_CALLS calls: look-back launches tag their tile
+ * descriptors with a 30-bit launch tag kept in the workspace, and tags
+ * repeat after 2^30 - 1 launches (one call launches at most 8 look-back
+ * kernels).  The workspace must be 8-byte aligned.
  *
  * The Ascend reference (huawei-csl/pytorch-tcuscan) launches AscendC
  * kernels `kernel(GM_ADDR in..., GM_ADDR out..., GM_ADDR workspace,
@@ -30,6 +35,9 @@
 #ifdef __cplusplus
 extern "C" {
 #endif
+
+/* Calls between re-zeroings of one workspace (see the workspace contract). */
+#define TCS_WORKSPACE_MAX_CALLS (1u << 24)
 
 /* Status codes. */
 enum tcs_status {
@@ -57,6 +65,24 @@ const char* tcs_status_string(int status);
 
 /* ABI version of this library (major * 100 + minor). */
 int tcs_abi_version(void);
+
+/* Zeroes a workspace on `stream` (a fill kernel, so it may be captured in a
+ * HIP graph): before its first use, and again at least once every
+ * TCS_WORKSPACE_MAX_CALLS calls.  This also restores the default poll
+ * limit.  No counterpart in the reference, which allocates a fresh
+ * workspace per call (src/torch/commons.h:25-55). */
+int tcs_workspace_reset(void* workspace, size_t workspace_bytes, void* stream);
+
+/* Look-back poll limit of launches on this workspace: a tile whose
+ * predecessor has not published after `polls` polls computes that
+ * predecessor's aggregate from the input itself (the forward-progress
+ * fallback).  0 falls back at the first poll that finds a predecessor
+ * missing, which exercises the fallback on every walk; a negative value
+ * restores the default (512).  Results do not depend on it.  Stream-ordered
+ * (a one-word write by a kernel).  No counterpart in the reference, whose
+ * multi-core kernels wait at SyncAll barriers instead
+ * (src/kernels/kernel_scan_multi_core.h:92,111,143). */
+int tcs_workspace_set_poll_limit(void* workspace, size_t workspace_bytes, int polls, void* stream);
 
 /* ------------------------------------------------------------------------
  * Scan family
@@ -88,7 +114,7 @@ enum tcs_scan_method {
   TCS_METHOD_AUTO = 0, /* the shipped method, the faster one measured on MI355X
                           (DPP for every dtype) */
   TCS_METHOD_DPP = 1,  /* lane-serial group sums joined by DPP wavefront scans */
-  TCS_METHOD_MFMA = 2  /* 32x32 units scanned on the matrix cores (F16, I8 only) */
+  TCS_METHOD_MFMA = 2  /* 32x32 units scanned on the matrix cores (F16, BF16, I8 only) */
 };
 
 /* tcs_scan with an explicit local scan method and look-back window
@@ -96,8 +122,10 @@ enum tcs_scan_method {
  * for MFMA); results are
  * identical for every choice (bit-exact).  TCS_METHOD_MFMA multiplies each 1024-element
  * unit, as 32 rows of 32, by a lower-triangular ones matrix on
- * v_mfma_f32_32x32x16_f16 (fp16, split into two exactly-summed parts) or
- * v_mfma_i32_32x32x32_i8 (int8), and is accepted for F16 and I8 inputs.
+ * v_mfma_f32_32x32x16_f16 (fp16, split into two exactly-summed parts),
+ * v_mfma_f32_32x32x16_bf16 (bf16, split into three exactly-summed bands
+ * of each row's exponent range) or v_mfma_i32_32x32x32_i8 (int8), and is
+ * accepted for F16, BF16 and I8 inputs.
  * Replaces: the Cube-unit row scan of scan_multi_core_{fp16,int8}
  * (src/kernels/kernel_row_scan.h:97-131 via
  * src/kernels/kernel_scan_multi_core.h:70-152) behind run_scan_multi_core
@@ -132,8 +160,8 @@ int tcs_seg_scan_flags(const void* x, int x_dtype, const int8_t* flags, void* y,
  * These keep the reference's block geometry.  reduce_tiles / complete_rows
  * split x into blocks [b*BL, min((b+1)*BL, n)) with BL = T * ceil(ceil(n/T)
  * / num_blocks) for a tile of T elements (scalar::GetWorkDistribution,
- * src/kernels/tcuscan_utils.h:1389-1408).  Every workspace's first 2 KiB
- * are reserved (zero between calls); *_workspace_bytes include them.
+ * src/kernels/tcuscan_utils.h:1389-1408).  Every workspace starts with a
+ * 4.5 KiB header of launch state; *_workspace_bytes include it.
  * --------------------------------------------------------------------- */
 
 size_t tcs_reduce_tiles_workspace_bytes(int64_t n, int64_t tile_len, int64_t num_blocks);

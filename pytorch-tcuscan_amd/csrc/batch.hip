@@ -93,7 +93,8 @@ std::vector<ScanCfg> candidates(const Op& op) {
   const int g = (op.x_dtype == TCS_F32 || op.x_dtype == TCS_I32) ? 4 : 8;
   const int64_t max_tile = 64 * 64 * g;
   std::vector<int> methods = {TCS_METHOD_DPP};
-  if (op.kind == kOpScan && (op.x_dtype == TCS_F16 || op.x_dtype == TCS_I8)) methods.push_back(TCS_METHOD_MFMA);
+  if (op.kind == kOpScan && (op.x_dtype == TCS_F16 || op.x_dtype == TCS_BF16 || op.x_dtype == TCS_I8))
+    methods.push_back(TCS_METHOD_MFMA);
   for (int S : {32, 64, 128}) {
     for (int tpw = 1; tpw <= 32; tpw *= 2) {
       int64_t tile = tcs::tile_elems_for(S, tpw);
@@ -124,6 +125,11 @@ struct tcs_batch {
   std::vector<hipStream_t> branches;
   std::vector<hipEvent_t> events;
   void* ws = nullptr;
+  size_t ws_bytes = 0;
+  // Look-back launches on the workspace since it was last zeroed (each op
+  // run or graph launch launches at most one per op region); the plan
+  // re-zeroes it before TCS_WORKSPACE_MAX_CALLS.
+  uint64_t launches = 0;
 
   void drop_graph() {
     if (exec) (void)hipGraphExecDestroy(exec);
@@ -173,6 +179,17 @@ int add_op(tcs_batch* b, Op op) {
 int check_ws(const tcs_batch* b, void* ws, size_t ws_bytes) {
   if (!ws) return TCS_ERR_WORKSPACE;
   if (ws_bytes < tcs_batch_workspace_bytes(b)) return TCS_ERR_WORKSPACE;
+  return TCS_OK;
+}
+
+// Counts `k` upcoming launches on ws; re-zeroes the plan's part of it on
+// the stream first when the count would reach the limit.
+int count_launches(tcs_batch* b, void* ws, uint64_t k, hipStream_t st) {
+  if (b->launches + k >= TCS_WORKSPACE_MAX_CALLS) {
+    if (int e = tcs_workspace_reset(ws, tcs_batch_workspace_bytes(b), st)) return e;
+    b->launches = 0;
+  }
+  b->launches += k;
   return TCS_OK;
 }
 
@@ -261,6 +278,7 @@ int tcs_batch_run_eager(tcs_batch* b, void* ws, size_t ws_bytes, void* stream) {
   if (!b) return TCS_ERR_INVALID_ARG;
   if (int e = check_ws(b, ws, ws_bytes)) return e;
   hipStream_t st = static_cast<hipStream_t>(stream);
+  if (int e = count_launches(b, ws, 1, st)) return e;
   for (const Op& op : b->ops)
     if (int e = run_op(op, op.cfg, static_cast<char*>(ws), st)) return e;
   return TCS_OK;
@@ -290,6 +308,7 @@ int tcs_batch_tune(tcs_batch* b, void* ws, size_t ws_bytes, int reps, int force,
     }
     ScanCfg best;
     for (ScanCfg c : candidates(op)) {
+      if ((rc = count_launches(b, ws, (uint64_t)reps + 1, st))) break;
       if ((rc = run_op(op, c, static_cast<char*>(ws), st))) break;  // warm-up
       if (hipEventRecord(e0, st) != hipSuccess) { rc = TCS_ERR_LAUNCH; break; }
       for (int r = 0; r < reps && !rc; ++r) rc = run_op(op, c, static_cast<char*>(ws), st);
@@ -377,11 +396,13 @@ int tcs_batch_instantiate(tcs_batch* b, int max_branches, void* ws, size_t ws_by
   b->graph = g;
   b->exec = ex;
   b->ws = ws;
+  b->ws_bytes = ws_bytes;
   return TCS_OK;
 }
 
 int tcs_batch_launch(tcs_batch* b, void* stream) {
   if (!b || !b->exec) return TCS_ERR_INVALID_ARG;
+  if (int e = count_launches(b, b->ws, 1, static_cast<hipStream_t>(stream))) return e;
   if (hipGraphLaunch(b->exec, static_cast<hipStream_t>(stream)) != hipSuccess) return TCS_ERR_LAUNCH;
   return TCS_OK;
 }

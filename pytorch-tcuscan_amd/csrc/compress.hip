@@ -104,7 +104,7 @@ __global__ __launch_bounds__(kCmpNW * kWave) void compress_kernel(
   using ST = typename Bits<Geom::kStageBytes>::T;
   __shared__ int s_wval[NW], s_wexcl[NW];
   __shared__ int64_t s_base;
-  __shared__ uint32_t s_tag;
+  __shared__ LaunchTag s_tag;
   __shared__ ST s_stage[NW][J * G * kWave];  // one wave's kept elements, in order
 
   if (threadIdx.x == 0) s_tag = launch_arrive(ws);
@@ -171,13 +171,13 @@ __global__ __launch_bounds__(kCmpNW * kWave) void compress_kernel(
       return FlagVal<int>{u == 0 ? 1 : 0, wave_sum(c)};
     };
     uint64_t* desc = ws_desc(ws);
-    const uint32_t tag = s_tag;
+    const uint32_t tag = s_tag.tag;
     int carry = 0;
     if (tile == 0) {
       if (lane == 0 && num_tiles > 1) desc_publish<int>(desc, 0, agg, tag, kFlagIncl);
     } else {
       if (lane == 0) desc_publish<int>(desc, tile, agg, tag, kFlagAgg);
-      carry = lookback<int>(desc, tile, tag, fb);
+      carry = lookback<int>(desc, tile, tag, s_tag.polls, fb);
       if (lane == 0) desc_publish<int>(desc, tile, carry + agg, tag, kFlagIncl);
     }
     // Counts are 32-bit: n < 2^31 per call (checked on the host).
@@ -295,7 +295,7 @@ __global__ __launch_bounds__((kCmpNW + 1) * kWave) void filter_kernel(
   using Regs = CmpRegs<XT, IT, J, G, kMask, IDX == 1>;
   __shared__ CmpSlot<NW> slot[2];
   __shared__ uint32_t s_draw[2];
-  __shared__ uint32_t s_tag;
+  __shared__ LaunchTag s_tag;
   __shared__ ST s_stage[NW][J * G * kWave];  // one wave's kept elements, in order
 
   const int lane = __lane_id();
@@ -347,9 +347,9 @@ __global__ __launch_bounds__((kCmpNW + 1) * kWave) void filter_kernel(
       sl.count = 0;
       uint64_t* desc = ws_desc(ws);
       if (t == 0) {
-        if (T > 1) desc_publish<int>(desc, 0, agg, s_tag, kFlagIncl);
+        if (T > 1) desc_publish<int>(desc, 0, agg, s_tag.tag, kFlagIncl);
       } else {
-        desc_publish<int>(desc, t, agg, s_tag, kFlagAgg);
+        desc_publish<int>(desc, t, agg, s_tag.tag, kFlagAgg);
       }
     }
   };
@@ -408,8 +408,8 @@ __global__ __launch_bounds__((kCmpNW + 1) * kWave) void filter_kernel(
     };
     int carry = 0;
     if (t > 0) {
-      carry = lookback<int>(ws_desc(ws), t, s_tag, fb);
-      if (lane == 0) desc_publish<int>(ws_desc(ws), t, carry + sl.agg, s_tag, kFlagIncl);
+      carry = lookback<int>(ws_desc(ws), t, s_tag.tag, s_tag.polls, fb);
+      if (lane == 0) desc_publish<int>(ws_desc(ws), t, carry + sl.agg, s_tag.tag, kFlagIncl);
     }
     if (lane == 0) sl.carry = carry;
   };
@@ -652,20 +652,17 @@ static int launch_compress(const void* x, const void* idx_in, Keep keep, void* o
     // Persistent grid: the resident workgroups, at most one per tile.
     constexpr int64_t kTile = FltGeom<XB, IB, IDX, true>::kTile;
     const uint32_t tiles = (uint32_t)((n + kTile - 1) / kTile);
-    static int per_cu[16] = {0};
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 16) dev = 0;
-    if (per_cu[dev] == 0) {
-      int b = 0;
+    static std::atomic<int> cached[kMaxCachedDevices];
+    uint32_t grid = (uint32_t)device_cached(cached, [](int dev) {
+      int b = 0, cus = 0;
       if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, filter_kernel<XB, IB, IDX, Keep>, (kCmpNW + 1) * kWave,
                                                        0) != hipSuccess ||
           b < 1)
         b = 1;
-      per_cu[dev] = b;
-    }
-    int cus = 0;
-    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 256;
-    uint32_t grid = (uint32_t)per_cu[dev] * (uint32_t)cus;
+      if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+        cus = 256;
+      return b * cus;
+    });
     if (grid > tiles) grid = tiles;
     hipLaunchKernelGGL((filter_kernel<XB, IB, IDX, Keep>), dim3(grid), dim3((kCmpNW + 1) * kWave), 0, st,
                        static_cast<const XT*>(x), static_cast<const IT*>(idx_in), keep, static_cast<XT*>(out_x),

@@ -20,6 +20,12 @@
 // deterministic.  complete_rows evaluates the row carries as a segmented
 // scan of the row-end values (one launch of the scan kernel) rather than
 // the reference's serial carry.
+//
+// The workspace scratch of these ops holds doubles and signed integers,
+// which a later look-back launch on the same workspace could read as
+// descriptors; each op zeroes its scratch before it returns (the last
+// reader of the partials clears them; complete_blocks / complete_rows end
+// with a fill).
 #include "tcs_common.hpp"
 
 namespace tcs {
@@ -52,12 +58,15 @@ __global__ __launch_bounds__(kRedBlock) void reduce_partials_kernel(
 }
 
 template <class Acc, class Out>
-__global__ void reduce_final_kernel(const Acc* __restrict__ partial, int64_t nb, int64_t cpb,
+__global__ void reduce_final_kernel(Acc* __restrict__ partial, int64_t nb, int64_t cpb,
                                     Out* __restrict__ out) {
   const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nb) return;
   Acc t = Acc(0);
-  for (int64_t c = 0; c < cpb; ++c) t = t + partial[b * cpb + c];
+  for (int64_t c = 0; c < cpb; ++c) {
+    t = t + partial[b * cpb + c];
+    partial[b * cpb + c] = Acc(0);
+  }
   out[b] = (Out)t;
 }
 
@@ -192,6 +201,7 @@ int tcs_complete_blocks(const void* x, int dtype, const void* sums, int64_t num_
   if (n % num_blocks != 0) return TCS_ERR_INVALID_ARG;
   if (dtype != TCS_F32 && dtype != TCS_I32) return TCS_ERR_UNSUPPORTED_DTYPE;
   if (!workspace || workspace_bytes < kHeaderBytes + (size_t)num_blocks * 8) return TCS_ERR_WORKSPACE;
+  if ((reinterpret_cast<uintptr_t>(workspace) & 7u) != 0) return TCS_ERR_WORKSPACE;
   if (n == 0) return TCS_OK;
   hipStream_t st = static_cast<hipStream_t>(stream);
   const int64_t bl = n / num_blocks;
@@ -211,7 +221,7 @@ int tcs_complete_blocks(const void* x, int dtype, const void* sums, int64_t num_
                        static_cast<const int32_t*>(x), pre, static_cast<int32_t*>(y), n, bl);
   }
   TCS_CHECK_LAUNCH();
-  return TCS_OK;
+  return fill_words(scratch, (size_t)num_blocks * 8, 0, st);
 }
 
 // complete_rows workspace: the nested scan's look-back state first (its
@@ -256,6 +266,7 @@ int tcs_complete_rows(const void* x, int dtype, const void* sums, int64_t num_su
   if (n == 0) return TCS_OK;
   const RowsPlan p = rows_plan(n, num_sums, tile_width, tile_height);
   if (!workspace || workspace_bytes < p.total) return TCS_ERR_WORKSPACE;
+  if ((reinterpret_cast<uintptr_t>(workspace) & 7u) != 0) return TCS_ERR_WORKSPACE;
   hipStream_t st = static_cast<hipStream_t>(stream);
   char* ws = static_cast<char*>(workspace);
   const uint32_t rgrid = (uint32_t)ceil_div(p.rows, 256);
@@ -293,7 +304,7 @@ int tcs_complete_rows(const void* x, int dtype, const void* sums, int64_t num_su
                        tile_width, p.block_len);
   }
   TCS_CHECK_LAUNCH();
-  return TCS_OK;
+  return fill_words(ws + p.pre_off, p.total - p.pre_off, 0, st);
 }
 
 }  // extern "C"

@@ -142,6 +142,7 @@ struct ScanArgs {
   uint32_t num_tiles;
   uint32_t tag;     // set in the kernel from the workspace epoch
   int window;       // look-back window (predecessors polled per step)
+  int polls;        // polls before the fallback (set in the kernel from the workspace)
 };
 
 // Element offset, within a data wave's part of the tile, of the G elements
@@ -286,9 +287,9 @@ __device__ __forceinline__ void tile_lookback(PipeSlot<C, NW>& sl,
   if (tile == 0) {
     if constexpr (!kSeg) carry = start;
   } else if (sl.aggf) {
-    if (!sl.first_head) carry = lookback<Acc>(desc, tile, a.tag, fb, a.window);
+    if (!sl.first_head) carry = lookback<Acc>(desc, tile, a.tag, a.polls, fb, a.window);
   } else {
-    carry = lookback<Acc>(desc, tile, a.tag, fb, a.window);
+    carry = lookback<Acc>(desc, tile, a.tag, a.polls, fb, a.window);
     if (lane == 0) desc_publish<Acc>(desc, tile, carry + sl.agg, a.tag, kFlagIncl);
   }
   if (lane == 0) sl.carry = carry;
@@ -494,7 +495,7 @@ __global__ __launch_bounds__((NW + 1) * kWave) void scan_kernel(
   constexpr int kStage = LOC == kLocMfma ? UnitScan<C>::kStageHalfs : 1;
   __shared__ PipeSlot<C, NW> slot[2];
   __shared__ uint32_t s_draw[2];
-  __shared__ uint32_t s_tag;
+  __shared__ LaunchTag s_tag;
   __shared__ __attribute__((aligned(16))) uint16_t stage_all[NW][kStage];
   __shared__ __attribute__((aligned(16))) uint32_t ostage_all[NW][OutStage<C, LOC>::kWords];
   const int lane = __lane_id();
@@ -524,7 +525,8 @@ __global__ __launch_bounds__((NW + 1) * kWave) void scan_kernel(
     s_draw[1] = d0 < a.num_tiles ? draw() : a.num_tiles;
   }
   __syncthreads();
-  a.tag = s_tag;
+  a.tag = s_tag.tag;
+  a.polls = s_tag.polls;
   uint32_t t = s_draw[0];
   uint32_t tn = s_draw[1];
   if (t >= a.num_tiles) return;
@@ -568,10 +570,8 @@ __global__ __launch_bounds__((NW + 1) * kWave) void scan_kernel(
 // device: the persistent grid size.
 template <class C, int NW, int J, int SEG, int LOC>
 static uint32_t resident_grid() {
-  static int cached[16] = {0};
-  int dev = 0;
-  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 16) dev = 0;
-  if (cached[dev] == 0) {
+  static std::atomic<int> cached[kMaxCachedDevices];
+  return (uint32_t)device_cached(cached, [](int dev) {
     int per_cu = 0, cus = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, scan_kernel<C, NW, J, SEG, LOC>,
                                                      (NW + 1) * kWave, 0) != hipSuccess ||
@@ -580,9 +580,8 @@ static uint32_t resident_grid() {
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
         cus < 1)
       cus = 256;
-    cached[dev] = per_cu * cus;
-  }
-  return (uint32_t)cached[dev];
+    return per_cu * cus;
+  });
 }
 
 // (NW, J) for a tile of `tile` elements and group width G.  The matrix-unit
@@ -600,7 +599,7 @@ static int launch_scan_tile(int64_t tile, const void* x, void* y, const int8_t* 
     uint32_t grid = resident_grid<C, NW, J, SEG, LOC>();                                         \
     if (grid > tiles) grid = tiles;                                                              \
     hipLaunchKernelGGL((scan_kernel<C, NW, J, SEG, LOC>), dim3(grid), dim3((NW + 1) * kWave), 0, \
-                       st, xi, yo, flags, start, ws, ScanArgs{n, seg_len, tiles, 0, window});  \
+                       st, xi, yo, flags, start, ws, ScanArgs{n, seg_len, tiles, 0, window, 0});  \
   } while (0)
   if constexpr (LOC == kLocMfma && UnitScan<C>::kGroups == 1) {
     switch (groups) {
@@ -662,7 +661,8 @@ static int scan_dispatch(const void* x, int x_dtype, void* y, const int8_t* flag
   if (window < 0 || window > kWave) return TCS_ERR_INVALID_ARG;
   if (method != TCS_METHOD_AUTO && method != TCS_METHOD_DPP && method != TCS_METHOD_MFMA)
     return TCS_ERR_INVALID_ARG;
-  if (method == TCS_METHOD_MFMA && (SEG != kSegNone || (x_dtype != TCS_F16 && x_dtype != TCS_I8)))
+  if (method == TCS_METHOD_MFMA &&
+      (SEG != kSegNone || (x_dtype != TCS_F16 && x_dtype != TCS_BF16 && x_dtype != TCS_I8)))
     return TCS_ERR_UNSUPPORTED_DTYPE;
   // The shipped local scan is the faster one measured on MI355X at
   // N = 2^28 with the default tiles: DPP for every dtype (fp16 5.6 vs 3.7
@@ -672,6 +672,10 @@ static int scan_dispatch(const void* x, int x_dtype, void* y, const int8_t* flag
   // at 6 vs 272 at 8, 284 at 16, 420 at 1; bf16 263 vs 270; int8 level from
   // 5 to 8), 32 for the matrix-unit scan.
   if (window == 0) window = method == TCS_METHOD_MFMA ? 32 : 6;
+  // An integer scan's start is an int32: reject what does not convert
+  // (NaN, infinities, values outside the int32 range) rather than cast it.
+  if ((x_dtype == TCS_I8 || x_dtype == TCS_I32) && !(start >= -2147483648.0 && start <= 2147483647.0))
+    return TCS_ERR_INVALID_ARG;
   if (n == 0) return TCS_OK;
   if (!x || !y || (SEG == kSegFlags && !flags)) return TCS_ERR_INVALID_ARG;
   if (SEG == kSegFixed && seg_len <= 0) return TCS_ERR_INVALID_ARG;
@@ -680,10 +684,12 @@ static int scan_dispatch(const void* x, int x_dtype, void* y, const int8_t* flag
   // for plain int8 scans (5.41 vs 5.09 TB/s at 32K), 32K for other 16-bit
   // and 8-bit inputs, 16K for 32-bit inputs (the 64-group maximum of 4-wide
   // groups); the matrix-unit flavour uses 16K for fp16 (at 32K its
-  // double-buffered tiles spill) and 32K for int8.
+  // double-buffered tiles spill), 8K for bf16 (its three band pieces spill
+  // at 16K) and 32K for int8.
   const bool i8_plain_dpp = SEG == kSegNone && method == TCS_METHOD_DPP && x_dtype == TCS_I8;
   int64_t tile = tiles_per_wg > 0 ? tile_elems_for(S, tiles_per_wg)
                  : (method == TCS_METHOD_MFMA && x_dtype == TCS_F16) ? 16384
+                 : (method == TCS_METHOD_MFMA && x_dtype == TCS_BF16) ? 8192
                  : (x_dtype == TCS_F32 || x_dtype == TCS_I32) ? 16384
                  : i8_plain_dpp ? 65536 : 32768;
   // Short inputs take smaller DPP tiles, so that more workgroups share

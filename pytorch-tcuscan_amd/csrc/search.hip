@@ -55,15 +55,12 @@ __global__ __launch_bounds__(kSsThreads) void searchsorted_kernel(const int32_t*
 // Four workgroups per CU of the current device (two are resident at a
 // time with 64 KiB of LDS each; the second pair evens out the tail).
 static uint32_t searchsorted_grid() {
-  static int cached[16] = {0};
-  int dev = 0;
-  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 16) dev = 0;
-  if (cached[dev] == 0) {
+  static std::atomic<int> cached[kMaxCachedDevices];
+  return (uint32_t)device_cached(cached, [](int dev) {
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 256;
-    cached[dev] = 4 * cus;
-  }
-  return (uint32_t)cached[dev];
+    return 4 * cus;
+  });
 }
 
 }  // namespace tcs

@@ -147,7 +147,7 @@ __global__ __launch_bounds__(kSegNW * kWave) void seg_reduce_kernel(
   __shared__ uint32_t s_rowk[kSegTile];       // at an end position: r - (r_lo - 1) of that row
   __shared__ Acc s_wval[NW], s_wexcl[NW], s_carry;
   __shared__ int s_wflag[NW], s_wexclf[NW];
-  __shared__ uint32_t s_tag;
+  __shared__ LaunchTag s_tag;
 
   // Every workgroup arrives, including those that return early below.
   if (threadIdx.x == 0) s_tag = launch_arrive(ws);
@@ -279,16 +279,17 @@ __global__ __launch_bounds__(kSegNW * kWave) void seg_reduce_kernel(
       return FlagVal<Acc>{s >= u0 ? 1 : 0, wave_sum(sum)};
     };
     uint64_t* desc = ws_desc(ws);
-    const uint32_t tag = s_tag;
+    const uint32_t tag = s_tag.tag;
+    const int max_polls = s_tag.polls;
     Acc carry = Acc(0);
     if (tile == 0) {
       if (lane == 0 && num_tiles > 1) desc_publish<Acc>(desc, 0, agg, tag, kFlagIncl);
     } else if (aggf) {
       if (lane == 0) desc_publish<Acc>(desc, tile, agg, tag, kFlagIncl);
-      if (!first_head) carry = lookback<Acc>(desc, tile, tag, fb);
+      if (!first_head) carry = lookback<Acc>(desc, tile, tag, max_polls, fb);
     } else {
       if (lane == 0) desc_publish<Acc>(desc, tile, agg, tag, kFlagAgg);
-      carry = lookback<Acc>(desc, tile, tag, fb);
+      carry = lookback<Acc>(desc, tile, tag, max_polls, fb);
       if (lane == 0) desc_publish<Acc>(desc, tile, carry + agg, tag, kFlagIncl);
     }
     if (lane == 0) s_carry = carry;

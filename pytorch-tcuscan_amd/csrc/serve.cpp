@@ -363,6 +363,7 @@ class Server {
       for (size_t k = 0; k < streams_.size(); ++k)
         ws_.push_back(at::zeros({(int64_t)need}, at::TensorOptions().dtype(at::kByte).device(at::kCUDA, device_)));
       ws_bytes_ = need;
+      ws_calls_.assign(streams_.size(), 0);
       hip_ok(hipDeviceSynchronize(), "workspace");
     }
     if (pinned_n_ < n_req) {
@@ -388,6 +389,13 @@ class Server {
     void* ws = ws_[k].data_ptr();
     const size_t wb = ws_bytes_;
     int rc = TCS_OK;
+    // Launch tags repeat after 2^30 - 1 look-back launches on a workspace:
+    // re-zero it (stream-ordered) every TCS_WORKSPACE_MAX_CALLS requests.
+    if (++ws_calls_[k] >= TCS_WORKSPACE_MAX_CALLS) {
+      rc = tcs_workspace_reset(ws, wb, st);
+      TORCH_CHECK(rc == TCS_OK, "Server: workspace reset failed: ", tcs_status_string(rc));
+      ws_calls_[k] = 0;
+    }
     switch (e.kind) {
       case kScan:
         rc = tcs_scan(e.a.data_ptr(), dcode(e.a.scalar_type()), out, e.n, 128, 0, 0.0, ws, wb, st);
@@ -426,6 +434,7 @@ class Server {
   std::vector<hipStream_t> streams_;
   std::vector<hipEvent_t> events_, free_ev_;
   std::vector<at::Tensor> ws_;
+  std::vector<uint64_t> ws_calls_;
   size_t ws_bytes_ = 0;
   std::vector<Entry> pool_;
   std::map<int64_t, at::Tensor> kept_;

@@ -168,7 +168,7 @@ __global__ __launch_bounds__(kSortNW * kWave) void sort_pass_kernel(
   __shared__ uint32_t s_excl[256];       // first tile-local position of each digit
   __shared__ int32_t s_gofs[256];        // output index of tile-local position 0 for each digit
   __shared__ uint32_t s_wsum[4];
-  __shared__ uint32_t s_tag;
+  __shared__ LaunchTag s_tag;
   __shared__ Raw s_keys[kSortTile];
   __shared__ int32_t s_vals[kSortTile];
   // Digits beside the reordered keys for keys of up to 2 bytes; 4-byte keys
@@ -235,7 +235,8 @@ __global__ __launch_bounds__(kSortNW * kWave) void sort_pass_kernel(
   __syncthreads();
 
   uint64_t* desc = ws_desc(ws);
-  const uint32_t tag = s_tag;
+  const uint32_t tag = s_tag.tag;
+  const int max_polls = s_tag.polls;
   uint32_t cnt = 0, incl = 0;
   if (tid < 256) {
     const int d = tid;
@@ -264,14 +265,14 @@ __global__ __launch_bounds__(kSortNW * kWave) void sort_pass_kernel(
     // nearest inclusive descriptor, one predecessor at a time (reading
     // several at once measured slower: the extra device-scope loads cost
     // more than the walk saves).  A predecessor that has not published
-    // after kFallbackPolls polls is counted from its keys here.
+    // after the poll limit (kFallbackPolls by default) is counted from its keys here.
     uint32_t prefix = 0;
     for (int64_t q = (int64_t)tile - 1; q >= 0; --q) {
       uint32_t v = 0;
       int flag = 0;
       int polls = 0;
       while (!desc_try_read<uint32_t>(desc, (uint64_t)q * 256 + d, tag, v, flag)) {
-        if (++polls > kFallbackPolls) {
+        if (polls++ >= max_polls) {
           const int64_t u0 = q * kSortTile;
           const int64_t u1 = u0 + kSortTile < n ? u0 + kSortTile : n;
           v = 0;

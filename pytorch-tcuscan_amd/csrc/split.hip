@@ -46,7 +46,7 @@ __global__ __launch_bounds__(kSplitNW * kWave) void split_kernel(
   constexpr int64_t GSPAN = (int64_t)kWave * G, WSEG = GSPAN * J;
   __shared__ int s_wval[NW], s_wexcl[NW];
   __shared__ int64_t s_base;
-  __shared__ uint32_t s_tag;
+  __shared__ LaunchTag s_tag;
   __shared__ XT s_stage[NW][WSEG];           // kept run, then the other run
   __shared__ int32_t s_istage[IDX ? NW : 1][IDX ? WSEG : 1];
 
@@ -106,13 +106,13 @@ __global__ __launch_bounds__(kSplitNW * kWave) void split_kernel(
       return FlagVal<int>{u == 0 ? 1 : 0, wave_sum(c)};
     };
     uint64_t* desc = ws_desc(ws);
-    const uint32_t tag = s_tag;
+    const uint32_t tag = s_tag.tag;
     int carry = 0;
     if (tile == 0) {
       if (lane == 0 && num_tiles > 1) desc_publish<int>(desc, 0, agg, tag, kFlagIncl);
     } else {
       if (lane == 0) desc_publish<int>(desc, tile, agg, tag, kFlagAgg);
-      carry = lookback<int>(desc, tile, tag, fb);
+      carry = lookback<int>(desc, tile, tag, s_tag.polls, fb);
       if (lane == 0) desc_publish<int>(desc, tile, carry + agg, tag, kFlagIncl);
     }
     if (lane == 0) s_base = carry;

@@ -6,6 +6,8 @@
 #include <stdint.h>
 #include <stddef.h>
 
+#include <atomic>
+
 #include "tcs_device.hpp"
 #include "../../include/tcuscan.h"
 
@@ -201,22 +203,31 @@ __device__ __forceinline__ void reg_barrier(Packed<T, N>& r) {
 // 256 bytes (one word each, apart so that their atomics do not serialise on
 // one address):
 //   slots 0-7    ticket counters of the persistent scan;
-//   slot 8       the launch epoch (see launch_arrive);
+//   slot 8       the launch epoch (see launch_arrive), and 128 bytes into
+//                the slot the look-back poll limit (0: the default);
 //   slot 9       groups arrived, slots 10-17: workgroups arrived per group.
-// All but the epoch are zero between calls (each launch resets them), and
-// no op puts scratch data in the header.  Look-back kernels keep 16 bytes
-// of descriptor per tile right after the header; other scratch follows the
-// descriptors.  One workspace serves many ops and sizes, so bytes that one
-// call uses as scratch a later call may read as descriptors: scratch is
-// written as 64-bit words whose high half is zero (counts, indices) or
-// that of a double or negative integer, none of which reads as a status
-// word carrying a current tag (tags are small: one per launch).
+// All but the epoch and the poll limit are zero between calls (each launch
+// resets them), and no op puts scratch data in the header.  Look-back
+// kernels keep 16 bytes of descriptor per tile right after the header;
+// other scratch follows the descriptors.  One workspace serves many ops and
+// sizes, so bytes that one call used as scratch a later call may read as
+// descriptors.  Two rules keep such bytes from ever reading as a current
+// descriptor:
+//   * scratch an op leaves behind has a zero high half in every 64-bit
+//     word (counts, row numbers, key digits); an op whose scratch holds
+//     other values (doubles, negative integers) zeroes it before it
+//     returns;
+//   * a reader accepts only status words with flag 1 or 2 carrying the
+//     current launch's tag, and tags repeat only after 2^30 - 1 launches
+//     on one workspace, so every owner re-zeroes its workspace well before
+//     that (TCS_WORKSPACE_MAX_CALLS calls, tcs_workspace_reset).
 // ---------------------------------------------------------------------------
 constexpr int kTicketCounters = 8;
 constexpr int kSlotEpoch = 8, kSlotGroupsArrived = 9, kSlotArrived0 = 10;
 constexpr size_t kSlotBytes = 256;
 constexpr size_t kHeaderBytes = (kSlotArrived0 + kTicketCounters) * kSlotBytes;
 constexpr size_t kDescBytes = 16;
+constexpr size_t kPollWordOffset = kSlotEpoch * kSlotBytes + 128;
 
 inline size_t lookback_ws_bytes(uint64_t num_tiles) {
   const size_t b = kHeaderBytes + (size_t)num_tiles * kDescBytes;
@@ -230,19 +241,31 @@ __device__ __forceinline__ uint64_t* ws_desc(void* ws) {
   return reinterpret_cast<uint64_t*>(reinterpret_cast<char*>(ws) + kHeaderBytes);
 }
 
-// Launch tag of the current launch on workspace `ws`, read by one thread of
-// every workgroup at its start.  The tag comes from the epoch word in the
-// workspace, so it is right under HIP graph replay too (a tag passed as a
-// kernel argument would be frozen into the graph).  Each workgroup then
-// arrives on its group's counter (group = blockIdx.x % K); the last arrival
-// of a group arrives on the groups counter, and the last group advances
-// the epoch for the next launch.  Every workgroup's epoch read has returned
-// before its arrival is issued (the wait below), so the advance comes after
-// all reads.  The atomics are relaxed: acquire/release at agent scope would
-// write back and invalidate L2 in every workgroup, which costs streaming
-// kernels with thousands of workgroups several times their run time.
-__device__ __forceinline__ uint32_t launch_arrive(void* ws) {
+// Launch state of the current launch on workspace `ws`: its tag and the
+// look-back poll limit, read by one thread of every workgroup at its start.
+// The tag comes from the epoch word in the workspace, so it is right under
+// HIP graph replay too (a tag passed as a kernel argument would be frozen
+// into the graph).  Each workgroup then arrives on its group's counter
+// (group = blockIdx.x % K); the last arrival of a group arrives on the
+// groups counter, and the last group advances the epoch for the next
+// launch.  Every workgroup's epoch read has returned before its arrival is
+// issued (the wait below), so the advance comes after all reads.  The
+// atomics are relaxed: acquire/release at agent scope would write back and
+// invalidate L2 in every workgroup, which costs streaming kernels with
+// thousands of workgroups several times their run time.
+//
+// The poll limit word holds limit + 1, or 0 for the default
+// (kFallbackPolls); the host sets it with tcs_workspace_set_poll_limit.
+struct LaunchTag {
+  uint32_t tag;
+  int polls;
+};
+
+__device__ __forceinline__ LaunchTag launch_arrive(void* ws) {
   const uint64_t e = __hip_atomic_load(ws_slot(ws, kSlotEpoch), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const uint64_t pw = __hip_atomic_load(
+      reinterpret_cast<uint64_t*>(reinterpret_cast<char*>(ws) + kPollWordOffset), __ATOMIC_RELAXED,
+      __HIP_MEMORY_SCOPE_AGENT);
   asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
   const uint32_t K = gridDim.x < (uint32_t)kTicketCounters ? gridDim.x : (uint32_t)kTicketCounters;
   const uint32_t g = blockIdx.x % K;
@@ -256,7 +279,8 @@ __device__ __forceinline__ uint32_t launch_arrive(void* ws) {
       __hip_atomic_store(ws_slot(ws, kSlotEpoch), e + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
   }
-  return tag_of_epoch((uint32_t)(e % 0x3FFFFFFFull));
+  const int polls = pw == 0 ? kFallbackPolls : (pw > (uint64_t)kFallbackPolls + 1 ? kFallbackPolls : (int)(pw - 1));
+  return LaunchTag{tag_of_epoch((uint32_t)(e % 0x3FFFFFFFull)), polls};
 }
 
 // Tile length (elements) for a look-back kernel: tiles_per_wg matrix tiles
@@ -277,6 +301,28 @@ inline int64_t tile_elems_for(int S, int tiles_per_wg) {
 }
 
 inline bool aligned16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
+
+// A launch-size figure of the current device, computed once per device by
+// compute(device) and cached in `cache` (kMaxCachedDevices entries of a
+// static array, zero = not yet computed).  Threads racing on a first call
+// compute and store the same value; devices past the array are not cached.
+constexpr int kMaxCachedDevices = 64;
+template <class F>
+inline int device_cached(std::atomic<int>* cache, F&& compute) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0) dev = 0;
+  if (dev >= kMaxCachedDevices) return compute(dev);
+  int v = cache[dev].load(std::memory_order_relaxed);
+  if (v == 0) {
+    v = compute(dev);
+    cache[dev].store(v, std::memory_order_relaxed);
+  }
+  return v;
+}
+
+// Fills `bytes` (a multiple of 8, 8-byte aligned) at p with 64-bit `value`
+// on the stream, by a kernel (status.hip).
+int fill_words(void* p, size_t bytes, uint64_t value, hipStream_t st);
 
 // The flag-segmented scan with an explicit look-back window (scan_flags.hip).
 int seg_scan_flags_cfg(const void* x, int x_dtype, const int8_t* flags, void* y, int64_t n, int S,

@@ -161,6 +161,13 @@ __host__ __device__ inline uint32_t tag_of_epoch(uint32_t epoch) { return epoch 
 // ---------------------------------------------------------------------------
 constexpr uint32_t kFlagAgg = 1, kFlagIncl = 2;
 
+// Only the two flags a publisher writes are accepted: a word with flag 0
+// or 3 is never a descriptor, whatever its tag bits.
+__device__ __forceinline__ bool status_flag_ok(uint32_t s) {
+  const uint32_t f = s & 3u;
+  return f == kFlagAgg || f == kFlagIncl;
+}
+
 template <class T> struct DescWords { static constexpr int value = sizeof(T) == 8 ? 2 : 1; };
 
 template <class T>
@@ -184,14 +191,14 @@ __device__ __forceinline__ bool desc_try_read(const uint64_t* desc, int64_t tile
     const uint64_t w0 = __hip_atomic_load(d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const uint64_t w1 = __hip_atomic_load(d + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const uint32_t s0 = (uint32_t)(w0 >> 32), s1 = (uint32_t)(w1 >> 32);
-    if (s0 != s1 || (s0 >> 2) != tag) return false;
+    if (s0 != s1 || (s0 >> 2) != tag || !status_flag_ok(s0)) return false;
     flag = (int)(s0 & 3u);
     v = __builtin_bit_cast(T, ((w1 & 0xffffffffull) << 32) | (w0 & 0xffffffffull));
     return true;
   } else {
     const uint64_t w0 = __hip_atomic_load(desc + tile, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const uint32_t s0 = (uint32_t)(w0 >> 32);
-    if ((s0 >> 2) != tag) return false;
+    if ((s0 >> 2) != tag || !status_flag_ok(s0)) return false;
     flag = (int)(s0 & 3u);
     v = __builtin_bit_cast(T, (uint32_t)w0);
     return true;
@@ -210,7 +217,10 @@ __device__ __forceinline__ bool desc_try_read(const uint64_t* desc, int64_t tile
 // kFallbackPolls polls, the wave computes that tile's aggregate itself from
 // the (read-only) input through `fallback(tile) -> {flag, value}` and
 // carries on, so a waiting tile can never wait on a tile that has not been
-// scheduled.
+// scheduled.  The limit is kFallbackPolls unless the workspace sets a
+// smaller one (tcs_workspace_set_poll_limit; 0 gives up at the first poll
+// that finds a needed predecessor missing, which exercises the fallback on
+// every walk).
 constexpr int kFallbackPolls = 512;
 constexpr int kLookbackWindow = 16;
 
@@ -221,7 +231,7 @@ struct FlagVal {
 };
 
 template <class T, class Fallback>
-__device__ T lookback(const uint64_t* desc, uint32_t tile, uint32_t tag, Fallback&& fallback,
+__device__ T lookback(const uint64_t* desc, uint32_t tile, uint32_t tag, int max_polls, Fallback&& fallback,
                       int win = kLookbackWindow) {
   const int lane = __lane_id();
   T acc = T(0);
@@ -250,7 +260,7 @@ __device__ T lookback(const uint64_t* desc, uint32_t tile, uint32_t tag, Fallbac
         if (incl) return acc;
         break;
       }
-      if (polls >= kFallbackPolls) {
+      if (polls >= max_polls) {
         // Nearest predecessor the walk still needs.
         const int m = __builtin_ctzll(need & ~valid);
         const FlagVal<T> fv = fallback(base - m);

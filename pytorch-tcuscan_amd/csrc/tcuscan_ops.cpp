@@ -65,21 +65,47 @@ void require_device(const at::Tensor& t, const char* op, const char* name) {
   TORCH_CHECK(t.is_cuda(), op, ": ", name, " must be a GPU tensor");
 }
 
-// Per-(device, stream) workspace, zeroed on allocation.
+// Per-(device, stream) workspace, zeroed on allocation and again every
+// TCS_WORKSPACE_MAX_CALLS calls (launch tags repeat after 2^30 - 1 look-back
+// launches on one workspace).  `polls` is the look-back poll limit set by
+// _set_lookback_poll_limit (-1: the default), re-applied whenever the
+// workspace is replaced or re-zeroed.
+struct Workspace {
+  at::Tensor t;
+  uint64_t calls = 0;
+  int polls = -1;
+};
 std::mutex g_ws_mu;
-std::map<std::pair<int, void*>, at::Tensor> g_ws;
+std::map<std::pair<int, void*>, Workspace> g_ws;
 
-std::pair<void*, size_t> workspace(const at::Tensor& like, size_t bytes) {
+Workspace& workspace_entry(const at::Tensor& like, size_t bytes) {
   const int dev = like.device().index();
   void* s = stream_of(like);
-  std::lock_guard<std::mutex> lock(g_ws_mu);
   auto& w = g_ws[{dev, s}];
-  if (!w.defined() || (size_t)w.numel() < bytes) {
+  bool fresh = false;
+  if (!w.t.defined() || (size_t)w.t.numel() < bytes) {
     size_t want = bytes < (1u << 20) ? (1u << 20) : bytes;
-    if (w.defined() && want < 2 * (size_t)w.numel()) want = 2 * (size_t)w.numel();
-    w = at::zeros({(int64_t)want}, at::TensorOptions().dtype(at::kByte).device(like.device()));
+    if (w.t.defined() && want < 2 * (size_t)w.t.numel()) want = 2 * (size_t)w.t.numel();
+    w.t = at::zeros({(int64_t)want}, at::TensorOptions().dtype(at::kByte).device(like.device()));
+    w.calls = 0;
+    fresh = true;
+  } else if (++w.calls >= TCS_WORKSPACE_MAX_CALLS) {
+    const at::hip::HIPGuardMasqueradingAsCUDA guard(like.device());
+    check(tcs_workspace_reset(w.t.data_ptr(), (size_t)w.t.numel(), s), "workspace reset");
+    w.calls = 0;
+    fresh = true;
   }
-  return {w.data_ptr(), (size_t)w.numel()};
+  if (fresh && w.polls >= 0) {
+    const at::hip::HIPGuardMasqueradingAsCUDA guard(like.device());
+    check(tcs_workspace_set_poll_limit(w.t.data_ptr(), (size_t)w.t.numel(), w.polls, s), "workspace poll limit");
+  }
+  return w;
+}
+
+std::pair<void*, size_t> workspace(const at::Tensor& like, size_t bytes) {
+  std::lock_guard<std::mutex> lock(g_ws_mu);
+  Workspace& w = workspace_entry(like, bytes);
+  return {w.t.data_ptr(), (size_t)w.t.numel()};
 }
 
 at::ScalarType scan_out_dtype(at::ScalarType t) {
@@ -115,10 +141,10 @@ void scan_rows_into(const at::Tensor& x, at::Tensor& y, int64_t n, int64_t row_l
 // src/torch/torch_scan.h:47-110
 at::Tensor run_scan_single_core(const at::Tensor& x_in, int S, double starting_sum) {
   const char* op = "run_scan_single_core";
-  require_device(x_in, op, "x");
   const auto t = x_in.scalar_type();
   TORCH_CHECK(t == at::kHalf || t == at::kFloat || t == at::kChar || t == at::kBFloat16, op,
               ": x must be float16, bfloat16, float32 or int8, got ", t);
+  require_device(x_in, op, "x");
   at::Tensor x = prep(x_in);
   at::Tensor z = at::empty({x.numel()}, x.options().dtype(scan_out_dtype(t)));
   scan_into(x, z, x.numel(), S, starting_sum, op);
@@ -126,10 +152,10 @@ at::Tensor run_scan_single_core(const at::Tensor& x_in, int S, double starting_s
 }
 
 at::Tensor multi_core_scan(const at::Tensor& x_in, int S, const char* op) {
-  require_device(x_in, op, "x");
   const auto t = x_in.scalar_type();
   TORCH_CHECK(t == at::kHalf || t == at::kChar || t == at::kBFloat16, op,
               ": x must be float16, bfloat16 or int8, got ", t);
+  require_device(x_in, op, "x");
   at::Tensor x = prep(x_in);
   at::Tensor z = at::empty({x.numel()}, x.options().dtype(scan_out_dtype(t)));
   scan_into(x, z, x.numel(), S, 0.0, op);
@@ -153,7 +179,6 @@ at::Tensor run_scan_multi_core_no_l2(const at::Tensor& x, int S) {
 at::Tensor run_scan_ex(const at::Tensor& x_in, int S, int64_t tiles_per_wg, const std::string& method,
                        double starting_sum, int64_t lookback_window) {
   const char* op = "run_scan_ex";
-  require_device(x_in, op, "x");
   TORCH_CHECK(S > 0, op, ": S must be positive");
   TORCH_CHECK(tiles_per_wg >= 0, op, ": tiles_per_wg must be non-negative");
   TORCH_CHECK(lookback_window >= 0 && lookback_window <= 64, op, ": lookback_window must be in [0, 64]");
@@ -168,6 +193,7 @@ at::Tensor run_scan_ex(const at::Tensor& x_in, int S, int64_t tiles_per_wg, cons
   const auto t = x_in.scalar_type();
   TORCH_CHECK(t == at::kHalf || t == at::kFloat || t == at::kChar || t == at::kBFloat16 || t == at::kInt,
               op, ": x must be float16, bfloat16, float32, int8 or int32, got ", t);
+  require_device(x_in, op, "x");
   at::Tensor x = prep(x_in);
   at::Tensor z = at::empty({x.numel()}, x.options().dtype(scan_out_dtype(t)));
   const int64_t n = x.numel();
@@ -184,11 +210,11 @@ at::Tensor run_scan_ex(const at::Tensor& x_in, int S, int64_t tiles_per_wg, cons
 at::Tensor run_scan_multi_cube(const at::Tensor& x_in, const at::Tensor& upper,
                                const at::Tensor& lower_strict) {
   const char* op = "run_scan_multi_cube";
-  require_device(x_in, op, "x");
   TORCH_CHECK(upper.dim() == 2 && upper.size(0) == upper.size(1), op, ": upper must be S x S");
   TORCH_CHECK(lower_strict.sizes() == upper.sizes(), op, ": lower_strict must be S x S");
   const auto t = x_in.scalar_type();
   TORCH_CHECK(t == at::kHalf || t == at::kBFloat16, op, ": x must be float16 or bfloat16, got ", t);
+  require_device(x_in, op, "x");
   at::Tensor x = prep(x_in);
   at::Tensor z = at::empty({x.numel()}, x.options().dtype(at::kFloat));
   scan_into(x, z, x.numel(), (int)upper.size(0), 0.0, op);
@@ -198,11 +224,11 @@ at::Tensor run_scan_multi_cube(const at::Tensor& x_in, const at::Tensor& upper,
 // src/torch/torch_scan.h:119-175: row-wise scan of [B, L]
 at::Tensor run_scan_batch(const at::Tensor& x_in, int S) {
   const char* op = "run_scan_batch";
-  require_device(x_in, op, "x");
   const auto t = x_in.scalar_type();
   TORCH_CHECK(t == at::kHalf || t == at::kFloat || t == at::kBFloat16 || t == at::kChar, op,
               ": x must be float16, bfloat16, float32 or int8, got ", t);
   TORCH_CHECK(x_in.dim() >= 1, op, ": x must have at least one dimension");
+  require_device(x_in, op, "x");
   at::Tensor x = prep(x_in);
   const int64_t L = x.size(-1);
   TORCH_CHECK(L > 0, op, ": rows must be non-empty");
@@ -215,11 +241,11 @@ at::Tensor run_scan_batch(const at::Tensor& x_in, int S) {
 // src/torch/torch_scan.h:312-351: scan of each S-row of x[M, S]
 at::Tensor run_row_scan(const at::Tensor& x_in, int S) {
   const char* op = "run_row_scan";
-  require_device(x_in, op, "x");
   const auto t = x_in.scalar_type();
   TORCH_CHECK(t == at::kHalf || t == at::kBFloat16 || t == at::kFloat || t == at::kChar, op,
               ": x must be float16, bfloat16, float32 or int8, got ", t);
   TORCH_CHECK(S > 0, op, ": S must be positive");
+  require_device(x_in, op, "x");
   at::Tensor x = prep(x_in);
   const int64_t M = x.dim() == 0 ? 1 : x.size(0);
   const int64_t n = M * (int64_t)S;
@@ -233,12 +259,12 @@ at::Tensor run_row_scan(const at::Tensor& x_in, int S) {
 at::Tensor run_block_scan(const at::Tensor& x_in, const at::Tensor& upper,
                           const at::Tensor& lower_strict) {
   const char* op = "run_block_scan";
-  require_device(x_in, op, "x");
   TORCH_CHECK(upper.dim() == 2 && upper.size(0) == upper.size(1), op, ": upper must be S x S");
   TORCH_CHECK(lower_strict.sizes() == upper.sizes(), op, ": lower_strict must be S x S");
   const auto t = x_in.scalar_type();
   TORCH_CHECK(t == at::kHalf || t == at::kBFloat16 || t == at::kFloat || t == at::kChar, op,
               ": x must be float16, bfloat16, float32 or int8, got ", t);
+  require_device(x_in, op, "x");
   at::Tensor x = prep(x_in);
   const int64_t s = upper.size(0);
   at::Tensor z = at::empty({x.numel()}, x.options().dtype(scan_out_dtype(t)));
@@ -249,8 +275,6 @@ at::Tensor run_block_scan(const at::Tensor& x_in, const at::Tensor& upper,
 // src/torch/torch_seg_ops.h:164-193 (and run_seg_scan_vec :46-73):
 // f[i] = 1 starts a segment at i; output float32.
 at::Tensor seg_scan_flags(const at::Tensor& x_in, const at::Tensor& f_in, int S, const char* op) {
-  require_device(x_in, op, "x");
-  require_device(f_in, op, "f");
   const auto t = x_in.scalar_type();
   TORCH_CHECK(t == at::kHalf || t == at::kBFloat16 || t == at::kFloat || t == at::kChar, op,
               ": x must be float16, bfloat16, float32 or int8, got ", t);
@@ -259,6 +283,8 @@ at::Tensor seg_scan_flags(const at::Tensor& x_in, const at::Tensor& f_in, int S,
               op, ": f must be int8, uint8 or bool");
   TORCH_CHECK(f_in.numel() == x_in.numel(), op, ": x and f must have the same length");
   TORCH_CHECK(S > 0, op, ": S must be positive");
+  require_device(x_in, op, "x");
+  require_device(f_in, op, "f");
   at::Tensor x = prep(x_in);
   at::Tensor f = prep(f_in.scalar_type() == at::kChar ? f_in : f_in.to(at::kChar));
   const int64_t n = x.numel();
@@ -335,11 +361,11 @@ at::Tensor run_scan_cpu(const at::Tensor& x) {
 // src/torch/torch_reduce.h:42-75
 at::Tensor run_reduce_tiles(const at::Tensor& x_in, int64_t tile_len, int64_t num_blocks) {
   const char* op = "run_reduce_tiles";
-  require_device(x_in, op, "x");
   const auto t = x_in.scalar_type();
   TORCH_CHECK(t == at::kHalf || t == at::kChar || t == at::kBFloat16 || t == at::kFloat || t == at::kInt,
               op, ": x must be float16, bfloat16, float32, int8 or int32, got ", t);
   TORCH_CHECK(tile_len > 0 && num_blocks > 0, op, ": tile_len and num_blocks must be positive");
+  require_device(x_in, op, "x");
   at::Tensor x = prep(x_in);
   at::Tensor z = at::empty({num_blocks}, x.options().dtype(scan_out_dtype(t)));
   auto ws = workspace(x, tcs_reduce_tiles_workspace_bytes(x.numel(), tile_len, num_blocks));
@@ -354,18 +380,26 @@ at::Tensor run_reduce_tiles(const at::Tensor& x_in, int64_t tile_len, int64_t nu
 at::Tensor run_complete_rows(const at::Tensor& x_in, const at::Tensor& sums_in, int tile_width,
                              int tile_height) {
   const char* op = "run_complete_rows";
-  require_device(x_in, op, "x");
-  require_device(sums_in, op, "sums");
+  // The reference accepts float32 and int64 (int64 runs its int32 kernel
+  // and the output is int32, torch_reduce.h:145-147,165-170); int32 is
+  // accepted too.  Integer inputs and sums are taken as int32 values (an
+  // int64 outside the int32 range wraps, as .to(torch.int32) does).
   const auto t = x_in.scalar_type();
-  TORCH_CHECK(t == at::kFloat || t == at::kInt, op, ": x must be float32 or int32, got ", t);
-  TORCH_CHECK(sums_in.scalar_type() == t, op, ": sums must have the dtype of x");
+  const auto st = sums_in.scalar_type();
+  TORCH_CHECK(t == at::kFloat || t == at::kInt || t == at::kLong, op,
+              ": x must be float32, int32 or int64, got ", t);
+  TORCH_CHECK(t == at::kFloat ? st == at::kFloat : (st == at::kInt || st == at::kLong), op,
+              ": sums must be float32 for float32 x and int32 or int64 for integer x, got ", st);
   TORCH_CHECK(tile_width > 0 && tile_height > 0, op, ": tile sizes must be positive");
   TORCH_CHECK(sums_in.numel() > 0, op, ": sums must not be empty");
-  at::Tensor x = prep(x_in), sums = prep(sums_in);
+  require_device(x_in, op, "x");
+  require_device(sums_in, op, "sums");
+  at::Tensor x = prep(t == at::kLong ? x_in.to(at::kInt) : x_in);
+  at::Tensor sums = prep(st == at::kLong ? sums_in.to(at::kInt) : sums_in);
   at::Tensor z = at::empty({x.numel()}, x.options());
   auto ws = workspace(x, tcs_complete_rows_workspace_bytes(x.numel(), sums.numel(), tile_width, tile_height));
   const at::hip::HIPGuardMasqueradingAsCUDA guard(x.device());
-  check(tcs_complete_rows(x.data_ptr(), dtype_code(t), sums.data_ptr(), sums.numel(), z.data_ptr(),
+  check(tcs_complete_rows(x.data_ptr(), dtype_code(x.scalar_type()), sums.data_ptr(), sums.numel(), z.data_ptr(),
                           x.numel(), tile_width, tile_height, ws.first, ws.second, stream_of(x)),
         op);
   return z;
@@ -374,14 +408,14 @@ at::Tensor run_complete_rows(const at::Tensor& x_in, const at::Tensor& sums_in, 
 // src/torch/torch_reduce.h:197-231
 at::Tensor run_complete_blocks(const at::Tensor& x_in, const at::Tensor& sums_in, int tile_length) {
   const char* op = "run_complete_blocks";
-  require_device(x_in, op, "x");
-  require_device(sums_in, op, "sums");
   const auto t = x_in.scalar_type();
   TORCH_CHECK(t == at::kFloat || t == at::kInt, op, ": x must be float32 or int32, got ", t);
   TORCH_CHECK(sums_in.scalar_type() == t, op, ": sums must have the dtype of x");
   TORCH_CHECK(tile_length > 0, op, ": tile_length must be positive");
   TORCH_CHECK(sums_in.numel() > 0 && x_in.numel() % sums_in.numel() == 0, op,
               ": len(x) must be a multiple of len(sums)");
+  require_device(x_in, op, "x");
+  require_device(sums_in, op, "sums");
   at::Tensor x = prep(x_in), sums = prep(sums_in);
   at::Tensor z = at::empty_like(x);
   auto ws = workspace(x, tcs_complete_blocks_workspace_bytes(sums.numel()));
@@ -410,14 +444,14 @@ int index_code(const at::Tensor& t) {
 // after the first), num_segments outputs.
 at::Tensor seg_sum_impl(const at::Tensor& x_in, const at::Tensor& idx_in, int64_t num_segments,
                         int64_t num_ends, const char* op) {
-  require_device(x_in, op, "x");
-  require_device(idx_in, op, "indptr");
   const auto t = x_in.scalar_type();
   TORCH_CHECK(t == at::kHalf || t == at::kChar || t == at::kBFloat16 || t == at::kFloat, op,
               ": x must be float16, bfloat16, float32 or int8, got ", t);
   TORCH_CHECK(index_code(idx_in) >= 0, op, ": indptr must be int32, uint32 or int64, got ",
               idx_in.scalar_type());
   TORCH_CHECK(num_segments >= 0, op, ": indptr is too short");
+  require_device(x_in, op, "x");
+  require_device(idx_in, op, "indptr");
   at::Tensor x = prep(x_in);
   at::Tensor idx = idx_in.contiguous();
   at::Tensor z = at::empty({num_segments}, x.options().dtype(scan_out_dtype(t)));
@@ -498,11 +532,11 @@ at::Tensor run_spmv_v2(const at::Tensor& vals_in, const at::Tensor& indptr_in,
               ": cols must be int32 or uint32, got ", cols_in.scalar_type());
   TORCH_CHECK(cols_in.numel() == vals_in.numel(), op, ": cols and vals must have the same length");
   TORCH_CHECK(s > 0, op, ": s must be positive");
+  check_segm_offsets(segm_offsets, op);
   require_device(vals_in, op, "vals");
   require_device(indptr_in, op, "indptr");
   require_device(cols_in, op, "cols");
   require_device(x_in, op, "x");
-  check_segm_offsets(segm_offsets, op);
   at::Tensor vals = prep(vals_in), cols = prep(cols_in), x = x_in.contiguous();
   at::Tensor indptr = indptr_in.contiguous();
   const int64_t R = indptr.numel() - 1;
@@ -538,13 +572,13 @@ at::Tensor run_spmv_v2_multi_cube(const at::Tensor& vals, const at::Tensor& indp
 at::Tensor run_seg_scan_mc_revert(const at::Tensor& scan_x_in, const at::Tensor& scan_f_in,
                                   const at::Tensor& diff_in) {
   const char* op = "run_seg_scan_mc_revert";
-  require_device(scan_x_in, op, "x");
-  require_device(scan_f_in, op, "f");
-  require_device(diff_in, op, "diff");
   TORCH_CHECK(scan_x_in.scalar_type() == at::kFloat, op, ": x must be float32, got ", scan_x_in.scalar_type());
   TORCH_CHECK(scan_f_in.scalar_type() == at::kInt, op, ": f must be int32, got ", scan_f_in.scalar_type());
   TORCH_CHECK(diff_in.scalar_type() == at::kFloat, op, ": diff must be float32, got ", diff_in.scalar_type());
   TORCH_CHECK(scan_f_in.numel() == scan_x_in.numel(), op, ": x and f must have the same length");
+  require_device(scan_x_in, op, "x");
+  require_device(scan_f_in, op, "f");
+  require_device(diff_in, op, "diff");
   at::Tensor sx = prep(scan_x_in), sf = prep(scan_f_in), d = diff_in.contiguous();
   const int64_t n = sx.numel();
   const at::hip::HIPGuardMasqueradingAsCUDA guard(sx.device());
@@ -560,12 +594,12 @@ at::Tensor run_seg_scan_mc_revert(const at::Tensor& scan_x_in, const at::Tensor&
 // the probabilities x sorted largest first and their inclusive mass.
 at::Tensor run_top_p_keep(const at::Tensor& sorted_in, const at::Tensor& mass_in, double p) {
   const char* op = "run_top_p_keep";
-  require_device(sorted_in, op, "probs_sort");
-  require_device(mass_in, op, "probs_sum");
   TORCH_CHECK(sorted_in.scalar_type() == at::kHalf, op, ": probs_sort must be float16, got ",
               sorted_in.scalar_type());
   TORCH_CHECK(mass_in.scalar_type() == at::kFloat, op, ": probs_sum must be float32, got ", mass_in.scalar_type());
   TORCH_CHECK(sorted_in.numel() == mass_in.numel(), op, ": probs_sort and probs_sum must have the same length");
+  require_device(sorted_in, op, "probs_sort");
+  require_device(mass_in, op, "probs_sum");
   at::Tensor x = prep(sorted_in), m = prep(mass_in);
   const at::hip::HIPGuardMasqueradingAsCUDA guard(x.device());
   at::Tensor kept = at::empty({x.numel()}, m.options());
@@ -640,12 +674,14 @@ void compress_into(const at::Tensor& x, const at::Tensor* idx_in, int idx_mode, 
 }
 
 void compress_checks(const at::Tensor& x, const at::Tensor& mask, int S, const char* op) {
-  require_device(x, op, "x");
-  require_device(mask, op, "mask");
-  TORCH_CHECK(mask.numel() == x.numel(), op, ": x and mask must have the same length");
-  TORCH_CHECK(S > 0, op, ": S must be positive");
   const auto es = x.element_size();
   TORCH_CHECK(es == 1 || es == 2 || es == 4 || es == 8, op, ": unsupported element size");
+  TORCH_CHECK(mask.scalar_type() == at::kChar || mask.scalar_type() == at::kByte || mask.scalar_type() == at::kBool,
+              op, ": mask must be int8 (uint8 and bool are accepted), got ", mask.scalar_type());
+  TORCH_CHECK(mask.numel() == x.numel(), op, ": x and mask must have the same length");
+  TORCH_CHECK(S > 0, op, ": S must be positive");
+  require_device(x, op, "x");
+  require_device(mask, op, "mask");
 }
 
 }  // namespace
@@ -677,10 +713,10 @@ std::tuple<at::Tensor, at::Tensor> run_compress_ind(const at::Tensor& x_in, cons
                                                     const at::Tensor& mask_in, int S) {
   const char* op = "run_compress_ind";
   compress_checks(x_in, mask_in, S, op);
-  require_device(idx_in_t, op, "indices_in");
   TORCH_CHECK(idx_in_t.numel() == x_in.numel(), op, ": indices_in and x must have the same length");
   TORCH_CHECK(idx_in_t.element_size() == 4 || idx_in_t.element_size() == 8, op,
               ": indices_in must have 4- or 8-byte elements");
+  require_device(idx_in_t, op, "indices_in");
   at::Tensor x = prep(x_in), mask = mask_i8(mask_in, op), idx = prep(idx_in_t);
   const int64_t k = count_ones(mask, op);
   at::Tensor z = at::empty({k}, x.options());
@@ -707,13 +743,13 @@ std::tuple<at::Tensor, at::Tensor> run_compress_ind_no_arange(const at::Tensor& 
 // reference differences a global prefix, which loses precision).
 at::Tensor run_seg_sum(const at::Tensor& x_in, const at::Tensor& f_in, int S) {
   const char* op = "run_seg_sum";
-  require_device(x_in, op, "x");
-  require_device(f_in, op, "f");
   TORCH_CHECK(S > 0, op, ": S must be positive");
   TORCH_CHECK(f_in.numel() == x_in.numel(), op, ": x and f must have the same length");
   const auto t = x_in.scalar_type();
   TORCH_CHECK(t == at::kHalf || t == at::kBFloat16 || t == at::kFloat || t == at::kChar, op,
               ": x must be float16, bfloat16, float32 or int8, got ", t);
+  require_device(x_in, op, "x");
+  require_device(f_in, op, "f");
   at::Tensor x = prep(x_in), f = mask_i8(f_in, op);
   const int64_t k = count_ones(f, op);
   at::Tensor ends = at::empty({k + 1}, x.options().dtype(at::kInt));
@@ -733,6 +769,38 @@ at::Tensor run_seg_sum(const at::Tensor& x_in, const at::Tensor& f_in, int S) {
 }
 
 int abi_version() { return tcs_abi_version(); }
+
+// Workspace of the current stream of like's device (created if needed), so
+// that tests can inspect and prepare its launch state.  The tensor shares
+// the workspace's memory.
+at::Tensor workspace_tensor(const at::Tensor& like) {
+  require_device(like, "_workspace_tensor", "like");
+  std::lock_guard<std::mutex> lock(g_ws_mu);
+  return workspace_entry(like, 0).t;
+}
+
+// Calls counted on that workspace since it was last zeroed; the next call
+// that finds TCS_WORKSPACE_MAX_CALLS re-zeroes it first.
+void workspace_set_calls(const at::Tensor& like, int64_t calls) {
+  require_device(like, "_workspace_set_calls", "like");
+  TORCH_CHECK(calls >= 0 && calls < (int64_t)TCS_WORKSPACE_MAX_CALLS, "_workspace_set_calls: out of range");
+  std::lock_guard<std::mutex> lock(g_ws_mu);
+  workspace_entry(like, 0).calls = (uint64_t)calls;
+}
+
+// Look-back poll limit of the workspace of the current stream of like's
+// device (0: every walk falls back at once to computing a missing
+// predecessor from the input; negative: the default).  Results do not
+// depend on it; tests use it to exercise the forward-progress fallback.
+void set_lookback_poll_limit(const at::Tensor& like, int polls) {
+  require_device(like, "_set_lookback_poll_limit", "like");
+  std::lock_guard<std::mutex> lock(g_ws_mu);
+  Workspace& w = workspace_entry(like, 0);
+  w.polls = polls < 0 ? -1 : polls;
+  const at::hip::HIPGuardMasqueradingAsCUDA guard(like.device());
+  check(tcs_workspace_set_poll_limit(w.t.data_ptr(), (size_t)w.t.numel(), w.polls, stream_of(like)),
+        "_set_lookback_poll_limit");
+}
 
 // ---------------------------------------------------------------------------
 // Batched work plans (no counterpart in the reference): the C ABI's
@@ -884,11 +952,11 @@ class BatchPlan {
 // (the reference reads a count back the same way), then one fused
 // compare-and-compact pass.
 at::Tensor filter_impl(const at::Tensor& x_in, double pivot, int cmp, bool values, int S, const char* op) {
-  require_device(x_in, op, "x");
   TORCH_CHECK(S > 0, op, ": S must be positive");
   const auto t = x_in.scalar_type();
   TORCH_CHECK(t == at::kHalf || t == at::kBFloat16 || t == at::kFloat, op,
               ": x must be float16, bfloat16 or float32, got ", t);
+  require_device(x_in, op, "x");
   at::Tensor x = prep(x_in);
   const int64_t n = x.numel();
   TORCH_CHECK(n < ((int64_t)1 << 31), op, ": at most 2^31 - 1 elements");
@@ -929,13 +997,13 @@ at::Tensor run_where(const at::Tensor& x, float pivot, int S) {
 
 std::pair<at::Tensor, at::Tensor> split_impl(const at::Tensor& x_in, const at::Tensor& mask_in,
                                              const at::Tensor* idx_in, int S, const char* op) {
-  require_device(x_in, op, "x");
-  require_device(mask_in, op, "mask");
   TORCH_CHECK(S > 0, op, ": S must be positive");
   TORCH_CHECK(mask_in.scalar_type() == at::kChar, op, ": mask must be int8, got ", mask_in.scalar_type());
   TORCH_CHECK(mask_in.numel() == x_in.numel(), op, ": mask and x must have the same length");
   const int eb = (int)x_in.element_size();
   TORCH_CHECK(eb == 1 || eb == 2 || eb == 4 || eb == 8, op, ": unsupported dtype ", x_in.scalar_type());
+  require_device(x_in, op, "x");
+  require_device(mask_in, op, "mask");
   at::Tensor x = prep(x_in), mask = prep(mask_in), idx;
   if (idx_in) {
     require_device(*idx_in, op, "indices_in");
@@ -973,10 +1041,10 @@ std::tuple<at::Tensor, at::Tensor> run_split_ind(const at::Tensor& x, const at::
 // ---------------------------------------------------------------------------
 
 std::tuple<at::Tensor, at::Tensor> sort_impl(const at::Tensor& x_in, bool descending, const char* op) {
-  require_device(x_in, op, "x");
   const int dt = dtype_code(x_in.scalar_type());
   TORCH_CHECK(dt >= 0 && tcs_sort_workspace_bytes(1, dt) > 0, op,
               ": x must be float16, bfloat16, float32, int8, int16, int32 or uint32, got ", x_in.scalar_type());
+  require_device(x_in, op, "x");
   at::Tensor x = prep(x_in);
   const int64_t n = x.numel();
   TORCH_CHECK(n < ((int64_t)1 << 31), op, ": at most 2^31 - 1 elements");
@@ -1008,10 +1076,10 @@ std::tuple<at::Tensor, at::Tensor> run_radix_sort_ex(const at::Tensor& x, bool d
 // ---------------------------------------------------------------------------
 
 std::tuple<at::Tensor, at::Tensor> topk_impl(const at::Tensor& x_in, int64_t k, bool largest, const char* op) {
-  require_device(x_in, op, "x");
   const int dt = dtype_code(x_in.scalar_type());
   TORCH_CHECK(dt >= 0 && tcs_topk_workspace_bytes(1, 1, dt) > 0, op,
               ": x must be float16, bfloat16, float32, int8, int16, int32 or uint32, got ", x_in.scalar_type());
+  require_device(x_in, op, "x");
   at::Tensor x = prep(x_in);
   const int64_t n = x.numel();
   TORCH_CHECK(n < ((int64_t)1 << 31), op, ": at most 2^31 - 1 elements");
@@ -1057,10 +1125,10 @@ std::tuple<at::Tensor, at::Tensor> run_topk(const at::Tensor& x, int64_t k, bool
 at::Tensor run_topk_pivot_fp16(const at::Tensor& x_in, int64_t k, int num_samples) {
   const char* op = "run_topk_pivot_fp16";
   (void)k;
-  require_device(x_in, op, "x");
   TORCH_CHECK(x_in.scalar_type() == at::kHalf, op, ": x must be float16, got ", x_in.scalar_type());
   TORCH_CHECK(num_samples >= 4 && num_samples <= 64, op, ": num_samples must be in [4, 64]");
   TORCH_CHECK(x_in.numel() > 0, op, ": x must not be empty");
+  require_device(x_in, op, "x");
   at::Tensor x = prep(x_in);
   const int64_t blocks = tcs_topk_pivot_blocks(x.numel(), num_samples);
   const at::hip::HIPGuardMasqueradingAsCUDA guard(x.device());
@@ -1073,12 +1141,12 @@ at::Tensor run_topk_pivot_fp16(const at::Tensor& x_in, int64_t k, int num_sample
 // reference's tile_len is a tiling hint with no counterpart here.
 at::Tensor run_count_if(const at::Tensor& x_in, float pivot, uint32_t tile_len, uint8_t compare_mode) {
   const char* op = "run_count_if";
-  require_device(x_in, op, "x");
   TORCH_CHECK(tile_len > 0, op, ": tile_len must be positive");
   TORCH_CHECK(compare_mode <= TCS_CMP_NE, op, ": compare_mode must be 0..5 (LT, GT, EQ, LE, GE, NE)");
   const auto t = x_in.scalar_type();
   TORCH_CHECK(t == at::kHalf || t == at::kBFloat16 || t == at::kFloat, op,
               ": x must be float16, bfloat16 or float32, got ", t);
+  require_device(x_in, op, "x");
   at::Tensor x = prep(x_in);
   const int64_t n = x.numel();
   auto ws = workspace(x, tcs_compress_workspace_bytes(n));
@@ -1094,11 +1162,11 @@ at::Tensor run_count_if(const at::Tensor& x_in, float pivot, uint32_t tile_len, 
 // src/torch/torch_searchsorted.h:33-82
 at::Tensor run_searchsorted(const at::Tensor& sorted_in, const at::Tensor& values_in) {
   const char* op = "run_searchsorted";
-  require_device(sorted_in, op, "sorted");
-  require_device(values_in, op, "values");
   TORCH_CHECK(sorted_in.scalar_type() == at::kInt, op, ": sorted must be int32, got ", sorted_in.scalar_type());
   TORCH_CHECK(values_in.scalar_type() == at::kInt, op, ": values must be int32, got ", values_in.scalar_type());
   TORCH_CHECK(sorted_in.numel() < ((int64_t)1 << 31), op, ": at most 2^31 - 1 sorted elements");
+  require_device(sorted_in, op, "sorted");
+  require_device(values_in, op, "values");
   at::Tensor sorted = sorted_in.contiguous(), values = values_in.contiguous();
   at::Tensor out = at::empty({values.numel()}, values.options());  // flat, as the reference returns
   const at::hip::HIPGuardMasqueradingAsCUDA guard(values.device());
@@ -1131,6 +1199,12 @@ PYBIND11_MODULE(tcuscan_ops, m) {
   namespace py = pybind11;
   m.doc() = "tcuscan operators for AMD Instinct MI355X (HIP, gfx950)";
   m.def("abi_version", &tcuscan::abi_version, "C ABI version of libtcuscan_hip.so");
+  m.def("_workspace_tensor", &tcuscan::workspace_tensor, py::arg("like"),
+        "The per-stream workspace of like's device (shares its memory)");
+  m.def("_workspace_set_calls", &tcuscan::workspace_set_calls, py::arg("like"), py::arg("calls"),
+        "Sets the call count of the per-stream workspace");
+  m.def("_set_lookback_poll_limit", &tcuscan::set_lookback_poll_limit, py::arg("like"), py::arg("polls"),
+        "Look-back poll limit of the per-stream workspace (0: always fall back; -1: default)");
   m.def("run_scan_multi_core", &tcuscan::run_scan_multi_core, "Multi-core Scan");
   m.def("run_scan_multi_core_no_l2", &tcuscan::run_scan_multi_core_no_l2,
         "Multi-core Scan (no L2 cache optimization)");

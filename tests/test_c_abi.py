@@ -74,7 +74,7 @@ def test_scan_method_argument_validation(lib):
     assert f(*args, -1, 0, fake, 1 << 24, None) == TCS_ERR_INVALID_ARG
     assert f(*args, 1, 65, fake, 1 << 24, None) == TCS_ERR_INVALID_ARG         # window > 64
     assert f(*args, 1, -2, fake, 1 << 24, None) == TCS_ERR_INVALID_ARG
-    for dt in (1, 2, 5):  # bf16, f32, i32: no matrix-unit path
+    for dt in (2, 5):  # f32, i32: no matrix-unit path
         assert f(fake, dt, fake, 1 << 20, 128, 0, 0.0, 2, 0, fake, 1 << 24, None) == TCS_ERR_UNSUPPORTED_DTYPE
     assert f(*args, 2, 0, fake, 256, None) == TCS_ERR_WORKSPACE
     assert f(fake, 0, fake, 0, 128, 0, 0.0, 2, 0, None, 0, None) == TCS_OK  # empty: no launch
@@ -95,3 +95,32 @@ def test_extension_rejects_cpu_tensors():
 
     with pytest.raises(RuntimeError, match="GPU tensor"):
         tcuscan_ops.run_scan_multi_core(torch.ones(4, dtype=torch.float16), 64)
+
+
+def test_integer_scan_rejects_unconvertible_start(lib):
+    """An int8 / int32 scan's starting sum must be a finite int32; the check
+    comes before the workspace check, so the null workspace below is never
+    reached (and nothing is launched)."""
+    f = lib.tcs_scan
+    f.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int,
+                  ctypes.c_int, ctypes.c_double, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]
+    fake = 0x100000
+    for dt in (3, 5):  # int8, int32
+        for start in (2.0 ** 31, -(2.0 ** 31) - 1, float("inf"), float("-inf"), float("nan"), 1e300):
+            assert f(fake, dt, fake, 1000, 128, 0, start, None, 0, None) == TCS_ERR_INVALID_ARG, (dt, start)
+        for start in (2.0 ** 31 - 1, -(2.0 ** 31), 0.0, -5.0):
+            assert f(fake, dt, fake, 1000, 128, 0, start, None, 0, None) == TCS_ERR_WORKSPACE, (dt, start)
+    # Float scans round start to float: every double is accepted.
+    assert f(fake, 0, fake, 1000, 128, 0, float("inf"), None, 0, None) == TCS_ERR_WORKSPACE
+
+
+def test_workspace_maintenance_validation(lib):
+    reset = lib.tcs_workspace_reset
+    reset.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]
+    polls = lib.tcs_workspace_set_poll_limit
+    polls.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_void_p]
+    fake = 0x100000
+    assert reset(None, 1 << 20, None) == TCS_ERR_WORKSPACE
+    assert reset(fake, 64, None) == TCS_ERR_WORKSPACE          # smaller than the header
+    assert polls(None, 1 << 20, 0, None) == TCS_ERR_WORKSPACE
+    assert polls(fake, 1 << 20, 513, None) == TCS_ERR_INVALID_ARG  # above the default limit

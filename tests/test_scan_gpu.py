@@ -93,11 +93,11 @@ def test_scan_rejects_bad_dtype(ops, dev):
         ops.run_scan_multi_core(torch.zeros(10, dtype=torch.float16, device=dev), 0)
 
 
-@pytest.mark.parametrize("dtype", [torch.float16, torch.int8], ids=str)
+@pytest.mark.parametrize("dtype", [torch.float16, torch.bfloat16, torch.int8], ids=str)
 def test_scan_full_size_2p28(ops, dev, dtype):
     """The large-scan workload at its full size, against the full oracle."""
     n = 1 << 28
-    if dtype == torch.float16:
+    if dtype != torch.int8:
         x = uniform_pm1(n, dtype, 0)
     else:
         x = torch.randint(-4, 5, (n,), generator=gen(0), dtype=torch.int8)

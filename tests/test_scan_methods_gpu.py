@@ -1,11 +1,12 @@
 """GPU parity of the two local scan methods (DPP wavefront scans and the
 matrix-unit scan of 32 x 32 units) against the oracle and each other.
 
-The matrix-unit method is accepted for fp16 and int8 plain scans.  Its
-results must be bit-identical to the oracle's (float64 prefix rounded once
-to float32; int32 with wrap) on every tile geometry it instantiates, on
-ragged lengths, on units that take its sequential path (|x| >= 256, inf,
-NaN) and with a starting sum.
+The matrix-unit method is accepted for fp16, bf16 and int8 plain scans.
+Its results must be bit-identical to the oracle's (float64 prefix rounded
+once to float32; int32 with wrap) on every tile geometry it instantiates, on
+ragged lengths, on units that take its sequential path (fp16 |x| >= 256;
+bf16 rows spanning more than its three bands, or reaching 2^122; inf, NaN)
+and with a starting sum.
 """
 import numpy as np
 import pytest
@@ -30,7 +31,7 @@ def _input(n, dtype, seed):
 
 @pytest.mark.parametrize("method", ["dpp", "mfma"])
 @pytest.mark.parametrize("tpw", TPW_S32)
-@pytest.mark.parametrize("dtype", [torch.float16, torch.int8], ids=str)
+@pytest.mark.parametrize("dtype", [torch.float16, torch.bfloat16, torch.int8], ids=str)
 def test_methods_every_geometry(ops, dev, method, tpw, dtype):
     for n in (1, 1000, 1024, 1025, 32768 * 3 + 517, 1_000_003):
         x = _input(n, dtype, n + tpw)
@@ -38,7 +39,7 @@ def test_methods_every_geometry(ops, dev, method, tpw, dtype):
         assert_exact(to_np(y), oracle.scan(to_np(x)), f"{method} tpw={tpw} n={n} {dtype}")
 
 
-@pytest.mark.parametrize("dtype", [torch.float16, torch.int8], ids=str)
+@pytest.mark.parametrize("dtype", [torch.float16, torch.bfloat16, torch.int8], ids=str)
 @pytest.mark.parametrize("n", [1, 31, 32, 33, 511, 512, 513, 1023, 1024, 1025, 2047, 2049,
                                16383, 16385, 65536 * 3 + 5])
 def test_mfma_ragged_lengths(ops, dev, dtype, n):
@@ -91,7 +92,7 @@ def test_mfma_inf_nan(ops, dev):
     assert np.isfinite(y2[:7777]).all() and np.isnan(y2[7777:]).all()
 
 
-@pytest.mark.parametrize("dtype", [torch.float16, torch.int8], ids=str)
+@pytest.mark.parametrize("dtype", [torch.float16, torch.bfloat16, torch.int8], ids=str)
 def test_mfma_starting_sum(ops, dev, dtype):
     n = 300_001
     x = _input(n, dtype, 41)
@@ -100,7 +101,7 @@ def test_mfma_starting_sum(ops, dev, dtype):
         assert_exact(to_np(y), oracle.scan(to_np(x), start), f"start={start}")
 
 
-@pytest.mark.parametrize("dtype", [torch.float16, torch.int8], ids=str)
+@pytest.mark.parametrize("dtype", [torch.float16, torch.bfloat16, torch.int8], ids=str)
 def test_methods_agree_at_scale(ops, dev, dtype):
     n = (1 << 26) + 12345
     x = _input(n, dtype, 51).to(dev)
@@ -118,8 +119,49 @@ def test_auto_method_and_rejections(ops, dev):
     x = uniform_pm1(100_000, torch.float16, 61).to(dev)
     assert torch.equal(ops.run_scan_ex(x, 128), ops.run_scan_multi_core(x, 128))
     with pytest.raises(RuntimeError):
-        ops.run_scan_ex(x.bfloat16(), 128, 0, "mfma")
-    with pytest.raises(RuntimeError):
         ops.run_scan_ex(x.float(), 128, 0, "mfma")
     with pytest.raises(RuntimeError):
+        ops.run_scan_ex(x.int(), 128, 0, "mfma")
+    with pytest.raises(RuntimeError):
         ops.run_scan_ex(x, 128, 0, "tensor-cores")
+
+
+# --- bf16 on the matrix units: three exponent bands per row ---------------
+def _bf16_from_bits(bits: torch.Tensor) -> torch.Tensor:
+    return bits.to(torch.int16).view(torch.bfloat16)
+
+
+def test_mfma_bf16_exponent_spans(ops, dev):
+    """Rows whose values span up to 28 binades below their largest stay on
+    the matrix cores; wider spans (bits below the lowest band) and tiny
+    values take the sequential path.  Every case is exact."""
+    n = 1 << 20
+    base = uniform_pm1(n, torch.float32, 71)
+    scale = torch.pow(2.0, -torch.randint(0, 40, (n,), generator=gen(72)).float())
+    x = (base * scale).bfloat16()
+    for tpw in (1, 8):
+        assert_exact(to_np(ops.run_scan_ex(x.to(dev), 32, tpw, "mfma")), oracle.scan(to_np(x)), f"tpw={tpw}")
+    # Every unit within 20 binades: all on the matrix cores.
+    narrow = (base * torch.pow(2.0, -torch.randint(0, 20, (n,), generator=gen(73)).float())).bfloat16()
+    assert_exact(to_np(ops.run_scan_ex(narrow.to(dev), 32, 8, "mfma")), oracle.scan(to_np(narrow)))
+
+
+def test_mfma_bf16_extremes(ops, dev):
+    """Subnormals, the smallest normals, values near 2^122 and 2^127 (rows
+    that could overflow fp32 row sums), inf and NaN."""
+    n = 1 << 18
+    # bf16 subnormals mixed with tiny normals.
+    bits = torch.randint(0, 0x0100, (n,), generator=gen(81), dtype=torch.int32)
+    sign = torch.randint(0, 2, (n,), generator=gen(82), dtype=torch.int32) << 15
+    tiny = _bf16_from_bits(bits | sign)
+    assert_exact(to_np(ops.run_scan_ex(tiny.to(dev), 64, 0, "mfma")), oracle.scan(to_np(tiny)), "subnormals")
+    # Large magnitudes: alternate signs so that prefix sums stay finite.
+    e = torch.randint(110, 127, (n,), generator=gen(83)).float()
+    big = (torch.pow(2.0, e) * torch.where(torch.arange(n) % 2 == 0, 1.0, -1.0)).bfloat16()
+    assert_exact(to_np(ops.run_scan_ex(big.to(dev), 64, 0, "mfma")), oracle.scan(to_np(big)), "large")
+    x = uniform_pm1(40000, torch.bfloat16, 84)
+    x[1500] = float("inf")
+    x[20000] = float("nan")
+    y = to_np(ops.run_scan_ex(x.to(dev), 32, 4, "mfma"))
+    assert_exact(y, oracle.scan(to_np(x)), "inf/nan")
+    assert np.isfinite(y[:1500]).all() and np.isinf(y[1500:20000]).all() and np.isnan(y[20000:]).all()
