@@ -45,7 +45,9 @@ enum tcs_status {
   TCS_ERR_INVALID_ARG = 1,       /* bad size, null pointer, misalignment */
   TCS_ERR_UNSUPPORTED_DTYPE = 2, /* dtype outside the op's accepted set */
   TCS_ERR_WORKSPACE = 3,         /* workspace missing or too small */
-  TCS_ERR_LAUNCH = 4             /* the HIP runtime rejected a launch */
+  TCS_ERR_LAUNCH = 4,            /* the HIP runtime rejected a launch */
+  TCS_ERR_SCRATCH_LIMIT = 5      /* input needs more scratch than the fixed
+                                    array of tcs_scan_noalloc holds */
 };
 
 /* Element type codes (match the tensor dtypes of the reference API). */
@@ -133,6 +135,26 @@ enum tcs_scan_method {
 int tcs_scan_method(const void* x, int x_dtype, void* y, int64_t n, int S, int tiles_per_wg,
                     double start, int method, int lookback_window, void* workspace,
                     size_t workspace_bytes, void* stream);
+
+/* Bytes of the fixed per-GPU scratch array of tcs_scan_noalloc. */
+#define TCS_NOALLOC_SCRATCH_BYTES (1u << 20)
+
+/* tcs_scan without a workspace argument, for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference,
+ * whose wrappers allocate a workspace per call (src/torch/commons.h:25-55).
+ * The look-back workspace is a file-scope device array of
+ * TCS_NOALLOC_SCRATCH_BYTES that every GPU holds until the process ends;
+ * the call allocates nothing (no hipMalloc, hipFree or allocating library
+ * routine).  Inputs whose workspace (tcs_scan_workspace_bytes(n, S,
+ * tiles_per_wg)) exceeds that size are rejected with
+ * TCS_ERR_SCRATCH_LIMIT before anything is launched: with the default
+ * tiles that is n above about 5.3e8 elements.  All calls on one GPU share
+ * its array, so calls that overlap on one GPU (other streams, other host
+ * threads) race; stream-ordered calls do not.  Same arguments, dtypes and
+ * results as tcs_scan otherwise. */
+int tcs_scan_noalloc(const void* x, int x_dtype, void* y, int64_t n, int S, int tiles_per_wg,
+                     double start, void* stream);
 
 /* Independent inclusive scans of consecutive rows of row_len elements
  * (the last row may be shorter).  Same dtypes as tcs_scan.

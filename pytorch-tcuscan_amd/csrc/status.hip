@@ -34,6 +34,7 @@ const char* tcs_status_string(int status) {
     case TCS_ERR_UNSUPPORTED_DTYPE: return "unsupported dtype";
     case TCS_ERR_WORKSPACE: return "workspace missing or too small";
     case TCS_ERR_LAUNCH: return "kernel launch failed";
+    case TCS_ERR_SCRATCH_LIMIT: return "input exceeds the fixed scratch of the no-allocation entry point";
     default: return "unknown status";
   }
 }
