@@ -6,9 +6,18 @@ tcuscan.run_scan_multi_core(x, S=128).  A step is one call over the whole
 input with the input already resident in HBM.  The metric is GB/s of
 algorithmic traffic, N * (2 + 4) bytes per step (the reference's formula).
 
+Timing.  `value` is the wall-clock rate of `--steps` back-to-back calls
+after `--warmup` untimed ones, bracketed by device synchronisations, with
+nothing else enqueued between the calls.  Separately, at least 100 calls
+are timed one by one with HIP events on the stream the ops launch on (warm
+caches), and 100 more after zeroing a 512 MiB buffer before each call
+(cold caches: the reference's method); mean, median and p99 of both are
+reported.
+
 Printed: one JSON line with the driver's fields plus
-  roofline      the scan kernel's average duration from HIP events on the
-                stream it runs on, as achieved GB/s against 8 TB/s;
+  roofline      the scan kernel's average duration over the event-timed
+                calls, as achieved GB/s against 8 TB/s, with its median,
+                p99 and the cache-flushed distribution;
   cpu_baseline  the oracle's C port of the reference's run_scan_cpu
                 (three-phase OpenMP blocked scan), timed on this host over
                 whole copies of the same workload; beside it the host (core
@@ -48,11 +57,26 @@ sys.path.insert(0, PKG)
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E peak
 
 
-def _timed_steps(fn, steps: int, warmup: int, spin_s: float = 0.25):
-    """Runs fn warmup+steps times; returns (wall seconds for the timed steps,
-    per-step event milliseconds).  Before the warm-up steps, fn runs for at
-    least `spin_s` seconds so the GPU is at its working clocks when timing
-    starts (a few short kernels after idle measure the clock ramp)."""
+FLUSH_BYTES = 512 << 20  # larger than the 256 MB Infinity Cache and the L2s together
+_flush_buf = None
+
+
+def _flush():
+    """Zeroes a 512 MiB device buffer: evicts the op's inputs from the L2s
+    and the Infinity Cache before a cold-cache timed call (the reference's
+    method, scripts/profiling/profile_tcuscan_ops.py:140-150)."""
+    import torch
+
+    global _flush_buf
+    if _flush_buf is None:
+        _flush_buf = torch.empty(FLUSH_BYTES // 4, dtype=torch.float32, device="cuda:0")
+    _flush_buf.zero_()
+
+
+def _spin(fn, spin_s: float):
+    """Runs fn for at least `spin_s` seconds so the GPU is at its working
+    clocks when timing starts (a few short kernels after idle measure the
+    clock ramp)."""
     import torch
 
     t_end = time.perf_counter() + spin_s
@@ -60,19 +84,69 @@ def _timed_steps(fn, steps: int, warmup: int, spin_s: float = 0.25):
         for _ in range(4):
             fn()
         torch.cuda.synchronize()
+
+
+def _timed_steps(fn, steps: int, warmup: int, spin_s: float = 0.25):
+    """Wall seconds of `steps` back-to-back calls of fn after `warmup`
+    untimed ones, bracketed by device synchronisations; nothing else is
+    enqueued between the calls."""
+    import torch
+
+    _spin(fn, spin_s)
     for _ in range(warmup):
         fn()
     torch.cuda.synchronize()
-    starts = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
-    ends = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
     t0 = time.perf_counter()
-    for i in range(steps):
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
+
+
+def _event_samples(fn, iters: int = 100, flush: bool = False):
+    """Per-call durations (ms) of `iters` calls, each bracketed by its own
+    HIP events on the current stream (where the ops launch their kernels).
+    flush=True zeroes the 512 MiB flush buffer before each call, outside
+    the events (cold caches, as the reference times its ops)."""
+    import torch
+
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(iters)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(iters)]
+    for i in range(iters):
+        if flush:
+            _flush()
         starts[i].record()
         fn()
         ends[i].record()
     torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    return wall, [s.elapsed_time(e) for s, e in zip(starts, ends)]
+    return [s.elapsed_time(e) for s, e in zip(starts, ends)]
+
+
+def _dist(ms):
+    """mean / median / p99 (nearest rank) of a list of durations."""
+    v = sorted(ms)
+    return {"mean_ms": sum(v) / len(v), "p50_ms": v[len(v) // 2],
+            "p99_ms": v[min(len(v) - 1, int(round(0.99 * len(v))) - 1)], "n": len(v)}
+
+
+def _measure(fn, steps: int, warmup: int, iters: int = 100, flush: bool = True, spin_s: float = 0.25):
+    """The timing every workload gets: wall seconds of `steps` back-to-back
+    calls, then `iters` event-timed calls with warm caches and, with flush,
+    `iters` more after a cache flush each."""
+    wall = _timed_steps(fn, steps, warmup, spin_s)
+    out = {"wall_s": wall, "steps": steps, "hot": _dist(_event_samples(fn, iters))}
+    if flush:
+        out["flushed"] = _dist(_event_samples(fn, iters, flush=True))
+    return out
+
+
+def _progress(msg: str):
+    """One line on stderr per workload (the JSON result goes to stdout)."""
+    print(f"bench.py: {msg}", file=sys.stderr, flush=True)
+
+
+def _rnd(d, k=4):
+    return {a: (round(b, k) if isinstance(b, float) else b) for a, b in d.items()}
 
 
 def _scan_workload(dtype_name: str, n: int, S: int, steps: int, warmup: int, method: str | None = None):
@@ -96,20 +170,21 @@ def _scan_workload(dtype_name: str, n: int, S: int, steps: int, warmup: int, met
         fn = lambda: tcuscan.run_scan_multi_core(x, S)  # noqa: E731
     else:
         fn = lambda: tcuscan.run_scan_ex(x, S, 0, method)  # noqa: E731
-    wall, ev_ms = _timed_steps(fn, steps, warmup)
-    ms = sorted(ev_ms)
-    avg_ms = sum(ev_ms) / len(ev_ms)
+    m = _measure(fn, steps, warmup, iters=max(100, steps))
     bytes_per_step = n * (in_b + out_b)
     del x
     torch.cuda.empty_cache()
+    hot = m["hot"]
     return {
         "bytes_per_step": bytes_per_step,
-        "wall_s": wall,
-        "event_avg_ms": avg_ms,
-        "event_p50_ms": ms[len(ms) // 2],
-        "event_p99_ms": ms[min(len(ms) - 1, int(0.99 * len(ms)))],
-        "gbs_wall": bytes_per_step * steps / wall / 1e9,
-        "gbs_kernel": bytes_per_step / (avg_ms * 1e-3) / 1e9,
+        "wall_s": m["wall_s"],
+        "event_avg_ms": hot["mean_ms"],
+        "event_p50_ms": hot["p50_ms"],
+        "event_p99_ms": hot["p99_ms"],
+        "event_n": hot["n"],
+        "flushed": m["flushed"],
+        "gbs_wall": bytes_per_step * steps / m["wall_s"] / 1e9,
+        "gbs_kernel": bytes_per_step / (hot["mean_ms"] * 1e-3) / 1e9,
     }
 
 
@@ -127,12 +202,13 @@ def _compress_workload(n: int, S: int, steps: int, warmup: int):
     idx = torch.arange(n, dtype=torch.int32, device=dev)
     k = int(mask.sum().item())
     torch.cuda.synchronize()
-    wall, ev_ms = _timed_steps(lambda: tcuscan.run_compress_ind(x, idx, mask, S), steps, warmup)
+    m = _measure(lambda: tcuscan.run_compress_ind(x, idx, mask, S), steps, warmup)
     bytes_per_step = 9 * n + 8 * k
     del x, mask, idx
     torch.cuda.empty_cache()
-    return {"bytes_per_step": bytes_per_step, "wall_s": wall, "event_avg_ms": sum(ev_ms) / len(ev_ms),
-            "gbs_wall": bytes_per_step * steps / wall / 1e9, "kept": k}
+    return {"bytes_per_step": bytes_per_step, "wall_s": m["wall_s"], "event_avg_ms": m["hot"]["mean_ms"],
+            "gbs_wall": bytes_per_step * steps / m["wall_s"] / 1e9, "kept": k, "hot": m["hot"],
+            "flushed": m["flushed"]}
 
 
 def _latency_workload(log2n: int = 20, calls: int = 2000):
@@ -158,9 +234,12 @@ def _latency_workload(log2n: int = 20, calls: int = 2000):
         torch.cuda.synchronize()
         lat.append(time.perf_counter() - t0)
     lat.sort()
-    wall, ev = _timed_steps(lambda: tcuscan.run_scan_single_core(x, 128), 500, 50, spin_s=0.1)
+    wall = _timed_steps(lambda: tcuscan.run_scan_single_core(x, 128), 500, 50, spin_s=0.1)
+    cold = _dist(_event_samples(lambda: tcuscan.run_scan_single_core(x, 128), 200, flush=True))
+    hot = _dist(_event_samples(lambda: tcuscan.run_scan_single_core(x, 128), 200))
     return {"p50_us": lat[len(lat) // 2] * 1e6, "p99_us": lat[int(0.99 * len(lat))] * 1e6,
-            "back_to_back_us": wall / 500 * 1e6, "gbs_back_to_back": 6 * n * 500 / wall / 1e9}
+            "back_to_back_us": wall / 500 * 1e6, "gbs_back_to_back": 6 * n * 500 / wall / 1e9,
+            "event_hot": hot, "event_flushed": cold}
 
 
 def _filter_workload(n: int, S: int, steps: int, warmup: int, pivot: float = 0.5):
@@ -177,12 +256,12 @@ def _filter_workload(n: int, S: int, steps: int, warmup: int, pivot: float = 0.5
     x = torch.randn(n, generator=g).half().to(dev)
     k = int((x >= pivot).sum().item())
     torch.cuda.synchronize()
-    wall, ev_ms = _timed_steps(lambda: tcuscan.run_filter_greater_eq(x, pivot, S), steps, warmup)
+    m = _measure(lambda: tcuscan.run_filter_greater_eq(x, pivot, S), steps, warmup, flush=False)
     bytes_per_step = 4 * n + 2 * k
     del x
     torch.cuda.empty_cache()
-    return {"bytes_per_step": bytes_per_step, "wall_s": wall, "event_avg_ms": sum(ev_ms) / len(ev_ms),
-            "gbs_wall": bytes_per_step * steps / wall / 1e9, "kept": k}
+    return {"bytes_per_step": bytes_per_step, "wall_s": m["wall_s"], "event_avg_ms": m["hot"]["mean_ms"],
+            "gbs_wall": bytes_per_step * steps / m["wall_s"] / 1e9, "kept": k}
 
 
 def _searchsorted_workload(len_log2: int, needles_log2: int, steps: int, warmup: int):
@@ -197,10 +276,10 @@ def _searchsorted_workload(len_log2: int, needles_log2: int, steps: int, warmup:
     a = torch.sort(torch.randint(-2**31, 2**31 - 1, (1 << len_log2,), generator=g, dtype=torch.int32)).values.to(dev)
     v = torch.randint(-2**31, 2**31 - 1, (1 << needles_log2,), generator=g, dtype=torch.int32).to(dev)
     torch.cuda.synchronize()
-    wall, ev_ms = _timed_steps(lambda: tcuscan.run_searchsorted(a, v), steps, warmup)
+    wall = _timed_steps(lambda: tcuscan.run_searchsorted(a, v), steps, warmup)
     del a, v
     torch.cuda.empty_cache()
-    return {"wall_s": wall, "event_avg_ms": sum(ev_ms) / len(ev_ms),
+    return {"wall_s": wall,
             "gneedles_per_s": (1 << needles_log2) * steps / wall / 1e9}
 
 
@@ -219,12 +298,12 @@ def _sort_workload(n: int, dtype_name: str, steps: int, warmup: int):
     else:
         x = torch.randint(-2**31, 2**31 - 1, (n,), generator=g, dtype=torch.int32).to(dev)
     torch.cuda.synchronize()
-    wall, ev_ms = _timed_steps(lambda: tcuscan.run_radix_sort(x, 128), steps, warmup)
-    twall, tev = _timed_steps(lambda: torch.sort(x, stable=True), max(3, steps // 2), 2, spin_s=0.1)
+    wall = _timed_steps(lambda: tcuscan.run_radix_sort(x, 128), steps, warmup)
+    twall = _timed_steps(lambda: torch.sort(x, stable=True), max(3, steps // 2), 2, spin_s=0.1)
     tsteps = max(3, steps // 2)
     del x
     torch.cuda.empty_cache()
-    return {"wall_s": wall, "event_avg_ms": sum(ev_ms) / len(ev_ms), "gkeys_per_s": n * steps / wall / 1e9,
+    return {"wall_s": wall, "gkeys_per_s": n * steps / wall / 1e9,
             "torch_sort_ms": twall * 1e3 / tsteps, "torch_gkeys_per_s": n * tsteps / twall / 1e9}
 
 
@@ -249,11 +328,11 @@ def _top_p_workload(n: int, steps: int, warmup: int, p: float = 0.1):
         return idx.gather(0, torch.multinomial(kept, 1))
 
     torch.cuda.synchronize()
-    wall, ev_ms = _timed_steps(lambda: tcuscan.sample_top_p(probs, p), steps, warmup)
-    twall, _ = _timed_steps(torch_top_p, steps, warmup, spin_s=0.1)
+    wall = _timed_steps(lambda: tcuscan.sample_top_p(probs, p), steps, warmup)
+    twall = _timed_steps(torch_top_p, steps, warmup, spin_s=0.1)
     del probs
     torch.cuda.empty_cache()
-    return {"wall_s": wall, "ms_per_call": wall * 1e3 / steps, "event_avg_ms": sum(ev_ms) / len(ev_ms),
+    return {"wall_s": wall, "ms_per_call": wall * 1e3 / steps,
             "torch_top_p_ms": twall * 1e3 / steps}
 
 
@@ -268,11 +347,11 @@ def _topk_workload(n: int, k: int, steps: int, warmup: int):
     g = torch.Generator(device="cpu").manual_seed(0)
     x = torch.randn(n, generator=g).half().to(dev)
     torch.cuda.synchronize()
-    wall, ev_ms = _timed_steps(lambda: tcuscan.run_topk_fp16(x, k, -10.0, 10.0, 128), steps, warmup)
-    twall, _ = _timed_steps(lambda: torch.topk(x, k), steps, warmup, spin_s=0.1)
+    wall = _timed_steps(lambda: tcuscan.run_topk_fp16(x, k, -10.0, 10.0, 128), steps, warmup)
+    twall = _timed_steps(lambda: torch.topk(x, k), steps, warmup, spin_s=0.1)
     del x
     torch.cuda.empty_cache()
-    return {"wall_s": wall, "ms_per_call": wall * 1e3 / steps, "event_avg_ms": sum(ev_ms) / len(ev_ms),
+    return {"wall_s": wall, "ms_per_call": wall * 1e3 / steps,
             "gbs_read_once": 2 * n * steps / wall / 1e9, "torch_topk_ms": twall * 1e3 / steps}
 
 
@@ -309,11 +388,13 @@ def _spmv_workload(rows: int, steps: int, warmup: int):
 
     vals, indptr, cols, x, nnz = spmv_matrix(rows)
     torch.cuda.synchronize()
-    wall, ev_ms = _timed_steps(lambda: tcuscan.run_spmv_v2(vals, indptr, cols, x, 128), steps, warmup)
-    per = wall / steps
+    m = _measure(lambda: tcuscan.run_spmv_v2(vals, indptr, cols, x, 128), steps, warmup)
+    per = m["wall_s"] / steps
     min_bytes = nnz * (2 + 4) + 4 * (rows + 1) + 2 * rows + 4 * rows
-    out = {"nnz": nnz, "wall_s": wall, "event_avg_ms": sum(ev_ms) / len(ev_ms),
-           "gflops": 2 * nnz / per / 1e9, "gbs_min_bytes": min_bytes / per / 1e9}
+    out = {"nnz": nnz, "wall_s": m["wall_s"], "event_avg_ms": m["hot"]["mean_ms"],
+           "gflops": 2 * nnz / per / 1e9, "gbs_min_bytes": min_bytes / per / 1e9,
+           "hot": m["hot"], "flushed": m["flushed"],
+           "gflops_flushed_p50": 2 * nnz / (m["flushed"]["p50_ms"] * 1e-3) / 1e9}
     del vals, indptr, cols, x
     torch.cuda.empty_cache()
     return out
@@ -327,8 +408,8 @@ def _copy_workload(n: int, steps: int, warmup: int):
     dev = torch.device("cuda:0")
     a = torch.rand(n, device=dev)
     b = torch.empty_like(a)
-    wall, ev_ms = _timed_steps(lambda: b.copy_(a), steps, warmup)
-    return {"gbs_wall": 8 * n * steps / wall / 1e9, "event_avg_ms": sum(ev_ms) / len(ev_ms)}
+    m = _measure(lambda: b.copy_(a), steps, warmup, flush=False)
+    return {"gbs_wall": 8 * n * steps / m["wall_s"] / 1e9, "event_avg_ms": m["hot"]["mean_ms"]}
 
 
 def _batched_workload(steps: int, warmup: int, num_scans: int = 10_000, num_segsums: int = 2_000):
@@ -355,7 +436,8 @@ def _batched_workload(steps: int, warmup: int, num_scans: int = 10_000, num_segs
     plan.tune(reps=2)
     tune_s = time.perf_counter() - t0
     plan.instantiate(max_branches=2)
-    wall_b, ev_b = _timed_steps(plan.launch, steps, warmup)
+    mb = _measure(plan.launch, steps, warmup, flush=False)
+    wall_b = mb["wall_s"]
 
     scan_pieces = sc.pieces()
     sum_pieces = ss.pieces()
@@ -367,7 +449,7 @@ def _batched_workload(steps: int, warmup: int, num_scans: int = 10_000, num_segs
             tcuscan.run_seg_sum_multi_core(xi, ipi, 128)
 
     pc_steps = max(2, min(steps, 5))
-    wall_p, _ = _timed_steps(per_call, pc_steps, 1)
+    wall_p = _timed_steps(per_call, pc_steps, 1)
     # Same results both ways (spot check of 50 scans and 50 seg-sums).
     rng = np.random.default_rng(0)
     same = True
@@ -389,7 +471,7 @@ def _batched_workload(steps: int, warmup: int, num_scans: int = 10_000, num_segs
     return {
         "scans": num_scans, "scan_elems": total, "segsums": num_segsums, "segsum_nnz": nnz,
         "segsum_rows": rows, "bytes_per_step": bytes_per_step,
-        "batched_ms": ms_b, "batched_event_avg_ms": sum(ev_b) / len(ev_b),
+        "batched_ms": ms_b, "batched_event_avg_ms": mb["hot"]["mean_ms"],
         "per_call_ms": ms_p, "speedup": ms_p / ms_b,
         "gbs_batched": bytes_per_step / (ms_b * 1e-3) / 1e9,
         "tuned": {"seg_scan": {"S": cfg[0][0], "tiles_per_wg": cfg[0][1], "method": cfg[0][2],
@@ -400,13 +482,14 @@ def _batched_workload(steps: int, warmup: int, num_scans: int = 10_000, num_segs
 
 def _serve_workload(seconds: float, rate: float, seed: int = 0):
     """The serving stream: open-loop Poisson arrivals at `rate` req/s for
-    `seconds`; mix 40% scans (fp16 and int8, N = 2^U(10, 26)), 30% CSR SpMV
-    (fp16, lognormal rows of mean 16 clipped to 20000, nnz 10^U(4, 7), at least
-    2^16 columns), 20% seg-sums (fp16,
-    geometric segments of mean 64, nnz 10^U(4, 7)), 10% compaction (fp32,
-    N = 2^U(10, 24), Bernoulli mask p ~ U(0.05, 0.9)).  Inputs are a
-    device-resident pool of distinct requests per kind (requests draw pool
-    entries uniformly); admission under a 64 GB HBM budget."""
+    `seconds`; mix 40% scans (fp16 and int8, N from 2^10 to 2^26), 30% CSR
+    SpMV (fp16, lognormal rows of mean 16 clipped to 20000, nnz from 10^4 to
+    10^7, at least 2^16 columns), 20% seg-sums (fp16, geometric segments of
+    mean 64, nnz from 10^4 to 10^7), 10% compaction (fp32, N from 2^10 to
+    2^24, Bernoulli mask p from 0.05 to 0.9).  Inputs are a device-resident
+    pool of distinct requests per kind whose sizes are log-spaced over each
+    range, both ends included (requests draw pool entries uniformly);
+    admission under a 64 GB HBM budget."""
     import numpy as np
     import torch
 
@@ -418,8 +501,9 @@ def _serve_workload(seconds: float, rate: float, seed: int = 0):
     srv = tcuscan.Server(64 << 30, 4, 32)
     kinds = {"scan": [], "spmv": [], "segsum": [], "compress": []}
     hold = []
-    for i in range(48):
-        n = int(2 ** rng.uniform(10, 26))
+    sizes = {"scan": 48, "spmv": 16, "segsum": 16, "compress": 16}
+    for i, lg in enumerate(np.linspace(10, 26, sizes["scan"])):
+        n = int(round(2 ** lg))
         g = torch.Generator(device=dev).manual_seed(1000 + i)
         if i % 2 == 0:
             x = (torch.rand(n, generator=g, device=dev) * 2 - 1).half()
@@ -427,23 +511,25 @@ def _serve_workload(seconds: float, rate: float, seed: int = 0):
             x = torch.randint(-128, 128, (n,), generator=g, device=dev, dtype=torch.int8)
         hold.append(x)
         kinds["scan"].append(srv.add_scan(x))
-    for i in range(16):
-        nnz = 10 ** rng.uniform(4, 7)
-        rows = max(16, int(nnz / 16))
+    spmv_nnz = []
+    for i, lg in enumerate(np.linspace(4, 7, sizes["spmv"])):
+        rows = max(16, int(round(10 ** lg / 16)))
         cols = max(rows, 1 << 16)  # room for distinct columns in the longest rows
         m = workloads.lognormal_csr(rows, cols, 16.0, 1.2, 20000, seed=2000 + i, device=dev)
         hold.append(m)
+        spmv_nnz.append(int(m[0].numel()))
         kinds["spmv"].append(srv.add_spmv(*m))
-    for i in range(16):
-        nnz = 10 ** rng.uniform(4, 7)
-        lg = float(np.log2(max(2.0, nnz / 64)))
-        ss = workloads.geometric_segsums(1, 64.0, lg, lg + 1e-6, seed=3000 + i, device=dev)
+    segsum_nnz = []
+    for i, lg in enumerate(np.linspace(4, 7, sizes["segsum"])):
+        lgs = float(np.log2(max(2.0, 10 ** lg / 64)))
+        ss = workloads.geometric_segsums(1, 64.0, lgs, lgs + 1e-6, seed=3000 + i, device=dev)
         x = ss.x.half()
         hold.append((x, ss.indptr))
+        segsum_nnz.append(int(x.numel()))
         kinds["segsum"].append(srv.add_seg_sum(x, ss.indptr))
-    for i in range(16):
-        n = int(2 ** rng.uniform(10, 24))
-        p = rng.uniform(0.05, 0.9)
+    for i, lg in enumerate(np.linspace(10, 24, sizes["compress"])):
+        n = int(round(2 ** lg))
+        p = float(np.linspace(0.05, 0.9, sizes["compress"])[(i * 7) % sizes["compress"]])
         g = torch.Generator(device=dev).manual_seed(4000 + i)
         x = torch.rand(n, generator=g, device=dev)
         mk = (torch.rand(n, generator=g, device=dev) < p).to(torch.int8)
@@ -467,6 +553,9 @@ def _serve_workload(seconds: float, rate: float, seed: int = 0):
     st["per_kind"] = per_kind
     st["offered_rps"] = rate
     st["seconds"] = seconds
+    st["pool"] = {"scan_n": [1 << 10, 1 << 26], "spmv_nnz": [min(spmv_nnz), max(spmv_nnz)],
+                  "segsum_nnz": [min(segsum_nnz), max(segsum_nnz)], "compress_n": [1 << 10, 1 << 24],
+                  "entries": {k: len(v) for k, v in kinds.items()}}
     del srv, hold
     torch.cuda.empty_cache()
     return st
@@ -605,8 +694,7 @@ def main() -> int:
     ap.add_argument("--S", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-extras", action="store_true", help="skip the extra workloads")
-    ap.add_argument("--serve-seconds", type=float, default=10.0,
-                    help="length of the serving stream (the full workload is 60 s)")
+    ap.add_argument("--serve-seconds", type=float, default=60.0, help="length of the serving stream")
     ap.add_argument("--serve-rate", type=float, default=2000.0, help="Poisson arrival rate, req/s")
     ap.add_argument("--only", choices=["scan_f16", "scan_i8", "scan_bf16", "scan_f16_mfma", "scan_i8_mfma",
                                        "scan_f16_dpp", "scan_i8_dpp", "compress_ind", "spmv", "copy_f32",
@@ -648,10 +736,12 @@ def main() -> int:
             r = _copy_workload(n, a.steps, a.warmup)
         print(json.dumps({"only": a.only, **{k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()}}))
         return 0
+    _progress(f"headline fp16 scan N=2^{a.log2n}")
     main_res = _scan_workload("fp16", n, a.S, a.steps, a.warmup)
     extras = {}
     if not a.no_extras:
         for name, method in (("int8", None), ("bf16", None), ("fp16", "mfma"), ("int8", "mfma")):
+            _progress(f"scan {name} {method or 'default'}")
             r = _scan_workload(name, n, a.S, max(10, a.steps // 2), max(3, a.warmup // 2), method)
             key = f"scan_{name}_2p{a.log2n}" + (f"_{method}" if method else "")
             extras[key] = {
@@ -659,22 +749,30 @@ def main() -> int:
                 "kernel_gbs": round(r["gbs_kernel"], 2),
                 "frac_of_peak": round(r["gbs_kernel"] / HBM_PEAK_GBS, 4),
                 "ms_per_step": round(r["wall_s"] * 1e3 / max(10, a.steps // 2), 4),
+                "event_p50_ms": round(r["event_p50_ms"], 4), "event_p99_ms": round(r["event_p99_ms"], 4),
+                "flushed": _rnd(r["flushed"]),
             }
         ksteps = max(10, a.steps // 2)
+        _progress("compress_ind")
         c = _compress_workload(n, a.S, ksteps, max(3, a.warmup // 2))
         extras[f"compress_ind_f32_2p{a.log2n}_p0.3"] = {
             "value": round(c["gbs_wall"], 2), "unit": "GB/s",
             "bytes_per_step": c["bytes_per_step"], "kept": c["kept"],
             "ms_per_step": round(c["wall_s"] * 1e3 / ksteps, 4),
+            "hot": _rnd(c["hot"]), "flushed": _rnd(c["flushed"]),
             "note": "includes the device-to-host count read that sizes the outputs",
         }
+        _progress("N=2^20 latency")
         lt = _latency_workload()
         extras["scan_f16_2p20_single_core_latency"] = {
             "value": round(lt["p50_us"], 2), "unit": "us p50 per synchronised call",
             "p99_us": round(lt["p99_us"], 2), "back_to_back_us": round(lt["back_to_back_us"], 2),
             "gbs_back_to_back": round(lt["gbs_back_to_back"], 1),
-            "note": "N = 2^20 fp16 -> fp32, S = 128; the 6 MiB working set sits in the caches",
+            "event_hot": _rnd(lt["event_hot"]), "event_flushed": _rnd(lt["event_flushed"]),
+            "note": "N = 2^20 fp16 -> fp32, S = 128; p50/p99 from the host around synchronised calls; "
+                    "event_* from HIP events around each call, event_flushed after a 512 MiB cache flush",
         }
+        _progress("filter")
         fl = _filter_workload(n, a.S, ksteps, max(3, a.warmup // 2))
         extras[f"filter_ge_f16_2p{a.log2n}_pivot0.5"] = {
             "value": round(fl["gbs_wall"], 2), "unit": "GB/s",
@@ -682,33 +780,41 @@ def main() -> int:
             "ms_per_step": round(fl["wall_s"] * 1e3 / ksteps, 4),
             "note": "count pass + fused compare-and-compact pass, with the count read back to size the output",
         }
+        _progress("radix sort")
         so = _sort_workload(n, "fp16", max(5, ksteps // 2), 2)
         extras[f"radix_sort_f16_2p{a.log2n}"] = {
             "value": round(so["gkeys_per_s"], 2), "unit": "G keys/s",
             "ms_per_step": round(so["wall_s"] * 1e3 / max(5, ksteps // 2), 4),
             "torch_sort_stable_ms": round(so["torch_sort_ms"], 4),
         }
+        _progress("top-k")
         tk = _topk_workload(n, 2048, ksteps, 2)
         extras[f"topk_f16_2p{a.log2n}_k2048"] = {
             "value": round(tk["ms_per_call"], 4), "unit": "ms per call",
             "torch_topk_ms": round(tk["torch_topk_ms"], 4),
         }
+        _progress("top-p")
         tp = _top_p_workload(1 << 24, ksteps, 2)
         extras["sample_top_p_f16_2p24"] = {
             "value": round(tp["ms_per_call"], 4), "unit": "ms per call",
             "torch_top_p_ms": round(tp["torch_top_p_ms"], 4),
         }
+        _progress("searchsorted")
         ss = _searchsorted_workload(24, 24, ksteps, max(3, a.warmup // 2))
         extras["searchsorted_i32_2p24_into_2p24"] = {
             "value": round(ss["gneedles_per_s"], 3), "unit": "G needles/s",
             "ms_per_step": round(ss["wall_s"] * 1e3 / ksteps, 4),
         }
+        _progress("spmv")
         sp = _spmv_workload(1 << 22, ksteps, max(3, a.warmup // 2))
         extras["spmv_v2_2p22_lognormal16"] = {
             "value": round(sp["gflops"], 2), "unit": "GFLOP/s", "nnz": sp["nnz"],
             "gbs_min_bytes": round(sp["gbs_min_bytes"], 2),
             "ms_per_step": round(sp["wall_s"] * 1e3 / ksteps, 4),
+            "hot": _rnd(sp["hot"]), "flushed": _rnd(sp["flushed"]),
+            "gflops_flushed_p50": round(sp["gflops_flushed_p50"], 2),
         }
+        _progress("batched plan")
         bt = _batched_workload(max(5, a.steps // 5), 2)
         extras["batched_uneven_10k_scans_2k_segsums"] = {
             "value": round(bt["speedup"], 2), "unit": "x vs per-call launches",
@@ -716,6 +822,7 @@ def main() -> int:
             "gbs_batched": round(bt["gbs_batched"], 1), "scan_elems": bt["scan_elems"],
             "segsum_nnz": bt["segsum_nnz"], "tuned": bt["tuned"], "same_results": bt["same_results"],
         }
+        _progress(f"serving stream ({a.serve_seconds:g} s)")
         sv = _serve_workload(a.serve_seconds, a.serve_rate)
         extras["serve_mixed_poisson"] = {
             "value": round(sv["p99_ms"], 3), "unit": "ms p99 latency",
@@ -723,9 +830,16 @@ def main() -> int:
             "throughput_rps": round(sv["throughput_rps"], 1), "offered_rps": sv["offered_rps"],
             "gbs": round(sv["gbs"], 2), "requests": sv["requests"], "seconds": sv["seconds"],
             "batches": sv["batches"], "per_kind": sv["per_kind"],
+            "peak_arena_bytes": sv["peak_arena_bytes"], "mean_queue_ms": round(sv["mean_queue_ms"], 4),
+            "host_us_per_batch": {"mean": round(sv["host_us_per_batch_mean"], 2),
+                                  "p50": round(sv["host_us_per_batch_p50"], 2),
+                                  "p99": round(sv["host_us_per_batch_p99"], 2)},
+            "host_us_per_request": round(sv["host_us_per_request"], 2),
+            "host_busy_frac": round(sv["host_busy_frac"], 4), "pool": sv["pool"],
         }
     cpu = None
     if not a.no_cpu:
+        _progress("CPU baselines")
         cpu = _cpu_baseline(n)
         cpu["host"] = _host_info()
         cpu["libraries"] = _cpu_library_baselines()
@@ -757,6 +871,8 @@ def main() -> int:
             "kernel_avg_ms": round(main_res["event_avg_ms"], 4),
             "kernel_p50_ms": round(main_res["event_p50_ms"], 4),
             "kernel_p99_ms": round(main_res["event_p99_ms"], 4),
+            "kernel_samples": main_res["event_n"],
+            "flushed": _rnd(main_res["flushed"]),
         },
         "cpu_baseline": cpu,
         "extras": extras,

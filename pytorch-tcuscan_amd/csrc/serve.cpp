@@ -219,7 +219,9 @@ class Server {
     std::vector<Flight> flights;
     std::vector<int> busy(nstreams_, 0);
     std::deque<int64_t> queue;
-    int64_t next = 0, finished = 0, batches = 0;
+    int64_t next = 0, finished = 0, batches = 0, host_req_n = 0;
+    std::vector<double> host_batch_us;
+    host_batch_us.reserve((size_t)N);
     size_t peak_used = 0;
     double bytes_total = 0;
     Arena arena(budget_);
@@ -259,6 +261,7 @@ class Server {
           queue.pop_front();
         }
         if (!batch.empty()) {
+          const double h0 = t;  // the dispatcher's own cost: admission, launches, event record
           peak_used = std::max(peak_used, arena.used());
           int k = 0;
           for (int j = 1; j < nstreams_; ++j)
@@ -271,9 +274,12 @@ class Server {
           }
           hipEvent_t ev = take_event();
           hip_ok(hipEventRecord(ev, streams_[k]), "event record");
+          const size_t nb = batch.size();
           flights.push_back({k, ev, std::move(batch)});
           ++busy[k];
           ++batches;
+          host_batch_us.push_back((now() - h0) * 1e6);
+          host_req_n += (int64_t)nb;
           progress = true;
         }
         if (!progress) {
@@ -315,6 +321,17 @@ class Server {
     out["throughput_rps"] = span > 0 ? N / span : 0.0;
     out["gbs"] = span > 0 ? bytes_total / span / 1e9 : 0.0;
     out["peak_arena_bytes"] = (int64_t)peak_used;
+    // Host time of the dispatcher per launched batch (admission, launches,
+    // event record), which every request of the batch waits for.
+    std::vector<double> hb = host_batch_us;
+    std::sort(hb.begin(), hb.end());
+    double hsum = 0;
+    for (double v : hb) hsum += v;
+    out["host_us_per_batch_mean"] = hb.empty() ? 0.0 : hsum / (double)hb.size();
+    out["host_us_per_batch_p50"] = hb.empty() ? 0.0 : hb[hb.size() / 2];
+    out["host_us_per_batch_p99"] = hb.empty() ? 0.0 : hb[std::min(hb.size() - 1, (size_t)(0.99 * hb.size()))];
+    out["host_us_per_request"] = host_req_n ? hsum / (double)host_req_n : 0.0;
+    out["host_busy_frac"] = span > 0 ? hsum * 1e-6 / span : 0.0;
     out["latency_s"] = lat;
     return out;
   }
