@@ -381,6 +381,56 @@ def spmv_matrix(rows: int, seed: int = 0):
     return vals, indptr, cols, x, nnz
 
 
+def _gather_ceiling(cols, x, parts: int, iters: int = 100):
+    """tcs_calib_gather_f16 over the SpMV matrix's own column indices and
+    vector: the random gathers alone (no products, rows or look-back), with
+    the gather split across the XCDs (parts = 2) or not; mean ms per call
+    over event-timed calls with warm caches."""
+    import torch
+
+    lib = ctypes.CDLL(os.path.join(PKG, "libtcuscan_hip.so"))
+    lib.tcs_calib_gather_out_elems.restype = ctypes.c_size_t
+    lib.tcs_calib_gather_out_elems.argtypes = [ctypes.c_int64, ctypes.c_int]
+    lib.tcs_calib_gather_f16.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_int64,
+                                         ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+    n = cols.numel()
+    out = torch.empty(lib.tcs_calib_gather_out_elems(n, parts), dtype=torch.float32, device=cols.device)
+
+    def call():
+        rc = lib.tcs_calib_gather_f16(cols.data_ptr(), n, x.data_ptr(), x.numel(), parts, out.data_ptr(),
+                                      ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
+        assert rc == 0, rc
+
+    _spin(call, 0.1)
+    return _dist(_event_samples(call, iters))["mean_ms"]
+
+
+def _stream_ceilings(gib: int = 2, iters: int = 50):
+    """tcs_calib_stream over `gib` GiB buffers: the read, write and copy
+    rates of HBM with nontemporal 16-byte accesses (GB/s, mean over
+    event-timed calls), the practical ceilings for streaming kernels."""
+    import torch
+
+    lib = ctypes.CDLL(os.path.join(PKG, "libtcuscan_hip.so"))
+    lib.tcs_calib_stream.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int,
+                                     ctypes.c_void_p]
+    nbytes = gib << 30
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda:0")
+    b = torch.empty_like(a)
+    out = {}
+    for mode, name, moved in ((0, "read", 1), (1, "write", 1), (2, "copy", 2)):
+        def call(mode=mode):
+            rc = lib.tcs_calib_stream(a.data_ptr(), b.data_ptr(), nbytes, mode,
+                                      ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
+            assert rc == 0, rc
+        _spin(call, 0.1)
+        ms = _dist(_event_samples(call, iters))["mean_ms"]
+        out[name] = round(moved * nbytes / (ms * 1e-3) / 1e9, 1)
+    del a, b
+    torch.cuda.empty_cache()
+    return out
+
+
 def _spmv_workload(rows: int, steps: int, warmup: int):
     import torch
 
@@ -394,7 +444,8 @@ def _spmv_workload(rows: int, steps: int, warmup: int):
     out = {"nnz": nnz, "wall_s": m["wall_s"], "event_avg_ms": m["hot"]["mean_ms"],
            "gflops": 2 * nnz / per / 1e9, "gbs_min_bytes": min_bytes / per / 1e9,
            "hot": m["hot"], "flushed": m["flushed"],
-           "gflops_flushed_p50": 2 * nnz / (m["flushed"]["p50_ms"] * 1e-3) / 1e9}
+           "gflops_flushed_p50": 2 * nnz / (m["flushed"]["p50_ms"] * 1e-3) / 1e9,
+           "gather_ceiling_ms": {"one_part": _gather_ceiling(cols, x, 1), "split": _gather_ceiling(cols, x, 2)}}
     del vals, indptr, cols, x
     torch.cuda.empty_cache()
     return out
@@ -412,14 +463,18 @@ def _copy_workload(n: int, steps: int, warmup: int):
     return {"gbs_wall": 8 * n * steps / m["wall_s"] / 1e9, "event_avg_ms": m["hot"]["mean_ms"]}
 
 
-def _batched_workload(steps: int, warmup: int, num_scans: int = 10_000, num_segsums: int = 2_000):
+def _batched_workload(steps: int, warmup: int, num_scans: int = 10_000, num_segsums: int = 2_000,
+                      per_call_leg: bool = True, tune_cache: str | None = None, branches: int = 2):
     """The batched uneven workload: `num_scans` fp16 scans with lengths
     log-uniform in [2^8, 2^20], concatenated with int8 start flags, plus
     `num_segsums` indptr seg-sums over int8 data (geometric segment lengths,
     mean 64; 2^U(4, 12) segments per problem).  The batched form is one
     BatchPlan (tuned, then captured as one HIP graph) launched per step; the
     per-call form launches run_scan_multi_core on every scan and
-    run_seg_sum_multi_core on every seg-sum problem, each its own tensor."""
+    run_seg_sum_multi_core on every seg-sum problem, each its own tensor.
+    With per_call_leg False only the graph replays run (for profiling);
+    with tune_cache the tuner's persisted cache is loaded from (and saved
+    to) that file, so a second run does no timing runs."""
     import numpy as np
     import torch
 
@@ -433,11 +488,27 @@ def _batched_workload(steps: int, warmup: int, num_scans: int = 10_000, num_segs
     y_scan = plan.add_seg_scan(sc.x, sc.flags)
     y_sum = plan.add_seg_sum(ss.x, ss.indptr)
     t0 = time.perf_counter()
-    plan.tune(reps=2)
+    if tune_cache:
+        tcuscan.tuning.tune(plan, reps=2, path=tune_cache)
+    else:
+        plan.tune(reps=2)
     tune_s = time.perf_counter() - t0
-    plan.instantiate(max_branches=2)
+    plan.instantiate(max_branches=branches)
     mb = _measure(plan.launch, steps, warmup, flush=False)
     wall_b = mb["wall_s"]
+
+    total = int(sc.offsets[-1])
+    nnz = int(ss.indptr_host[-1])
+    rows = int(ss.row_off[-1])
+    bytes_per_step = 7 * total + nnz + 4 * (rows + 1) + 4 * rows
+    cfg = plan.configs()
+    tuned = {"seg_scan": {"S": cfg[0][0], "tiles_per_wg": cfg[0][1], "method": cfg[0][2], "window": cfg[0][3],
+                          "ms": round(cfg[0][4], 4)}}
+    ms_b = wall_b * 1e3 / steps
+    if not per_call_leg:
+        return {"scans": num_scans, "scan_elems": total, "segsums": num_segsums, "segsum_nnz": nnz,
+                "bytes_per_step": bytes_per_step, "batched_ms": ms_b, "batched_event": mb["hot"],
+                "gbs_batched": bytes_per_step / (ms_b * 1e-3) / 1e9, "tuned": tuned, "tune_s": tune_s}
 
     scan_pieces = sc.pieces()
     sum_pieces = ss.pieces()
@@ -459,14 +530,8 @@ def _batched_workload(steps: int, warmup: int, num_scans: int = 10_000, num_segs
     for i in rng.choice(num_segsums, 50, replace=False):
         r0, r1 = ss.row_off[i], ss.row_off[i + 1]
         same &= bool(torch.equal(tcuscan.run_seg_sum_multi_core(*sum_pieces[i], 128), y_sum[r0:r1]))
-    total = int(sc.offsets[-1])
-    nnz = int(ss.indptr_host[-1])
-    rows = int(ss.row_off[-1])
-    bytes_per_step = 7 * total + nnz + 4 * (rows + 1) + 4 * rows
-    cfg = plan.configs()
     del sc, ss, plan, scan_pieces, sum_pieces, y_scan, y_sum
     torch.cuda.empty_cache()
-    ms_b = wall_b * 1e3 / steps
     ms_p = wall_p * 1e3 / pc_steps
     return {
         "scans": num_scans, "scan_elems": total, "segsums": num_segsums, "segsum_nnz": nnz,
@@ -474,9 +539,7 @@ def _batched_workload(steps: int, warmup: int, num_scans: int = 10_000, num_segs
         "batched_ms": ms_b, "batched_event_avg_ms": mb["hot"]["mean_ms"],
         "per_call_ms": ms_p, "speedup": ms_p / ms_b,
         "gbs_batched": bytes_per_step / (ms_b * 1e-3) / 1e9,
-        "tuned": {"seg_scan": {"S": cfg[0][0], "tiles_per_wg": cfg[0][1], "method": cfg[0][2],
-                               "window": cfg[0][3], "ms": round(cfg[0][4], 4)}},
-        "tune_s": tune_s, "same_results": same,
+        "tuned": tuned, "tune_s": tune_s, "same_results": same,
     }
 
 
@@ -696,10 +759,13 @@ def main() -> int:
     ap.add_argument("--no-extras", action="store_true", help="skip the extra workloads")
     ap.add_argument("--serve-seconds", type=float, default=60.0, help="length of the serving stream")
     ap.add_argument("--serve-rate", type=float, default=2000.0, help="Poisson arrival rate, req/s")
+    ap.add_argument("--branches", type=int, default=2, help="with --only batched: graph branches of the plan")
+    ap.add_argument("--tune-cache", default=None,
+                    help="with --only batched: load the batch tuner's cache from this JSON file (and save it)")
     ap.add_argument("--only", choices=["scan_f16", "scan_i8", "scan_bf16", "scan_f16_mfma", "scan_i8_mfma",
                                        "scan_f16_dpp", "scan_i8_dpp", "compress_ind", "spmv", "copy_f32",
                                        "batched", "serve", "filter_ge", "searchsorted", "sort_f16", "sort_i32",
-                                       "topk_f16", "top_p"],
+                                       "topk_f16", "top_p", "stream_ceilings"],
                     help="run one workload and print its timing only (for profilers)")
     a = ap.parse_args()
     if a.gpus != 1:
@@ -718,8 +784,11 @@ def main() -> int:
             r = _compress_workload(n, a.S, a.steps, a.warmup)
         elif a.only == "spmv":
             r = _spmv_workload(1 << 22, a.steps, a.warmup)
+        elif a.only == "stream_ceilings":
+            r = _stream_ceilings()
         elif a.only == "batched":
-            r = _batched_workload(a.steps, a.warmup)
+            r = _batched_workload(a.steps, a.warmup, per_call_leg=False, tune_cache=a.tune_cache,
+                                  branches=a.branches)
         elif a.only == "serve":
             r = _serve_workload(a.serve_seconds, a.serve_rate)
         elif a.only == "filter_ge":
@@ -813,6 +882,10 @@ def main() -> int:
             "ms_per_step": round(sp["wall_s"] * 1e3 / ksteps, 4),
             "hot": _rnd(sp["hot"]), "flushed": _rnd(sp["flushed"]),
             "gflops_flushed_p50": round(sp["gflops_flushed_p50"], 2),
+            "gather_ceiling_ms": _rnd(sp["gather_ceiling_ms"]),
+            "frac_of_gather_ceiling": round(min(sp["gather_ceiling_ms"].values()) / sp["hot"]["mean_ms"], 4),
+            "note": "gather_ceiling_ms: the matrix's random gathers of x alone (calibration kernel), unsplit "
+                    "and split across the XCDs by column half",
         }
         _progress("batched plan")
         bt = _batched_workload(max(5, a.steps // 5), 2)
@@ -844,6 +917,8 @@ def main() -> int:
         cpu["host"] = _host_info()
         cpu["libraries"] = _cpu_library_baselines()
 
+    _progress("streaming ceilings")
+    ceilings = _stream_ceilings()
     ms_per_step = main_res["wall_s"] * 1e3 / a.steps
     out = {
         "metric": "scan GB/s & %HBM peak (N=2^28 fp16/int8); SpMV GFLOP/s; p50/p99 mixed-stream",
@@ -873,6 +948,9 @@ def main() -> int:
             "kernel_p99_ms": round(main_res["event_p99_ms"], 4),
             "kernel_samples": main_res["event_n"],
             "flushed": _rnd(main_res["flushed"]),
+            "stream_ceilings_gbs": ceilings,
+            "note": "stream_ceilings_gbs: HBM read / write / copy rates of a calibration kernel (nontemporal "
+                    "16-byte accesses), the practical ceilings beside the 8 TB/s peak",
         },
         "cpu_baseline": cpu,
         "extras": extras,

@@ -243,11 +243,20 @@ int tcs_seg_sum_indptr(const void* x, int x_dtype, const void* ends, int ends_dt
                        int64_t num_segments, int64_t num_ends, void* y, int64_t nnz,
                        void* workspace, size_t workspace_bytes, void* stream);
 
+/* Workspace bytes for tcs_spmv_csr with this shape: room for the split
+ * gather (vectors above 2 MiB: two look-back chains and two double partial
+ * sums per row).  Any workspace of at least tcs_seg_sum_workspace_bytes(nnz)
+ * is accepted; one too small for the split runs the single-chain gather,
+ * with identical results. */
+size_t tcs_spmv_workspace_bytes(int64_t nnz, int64_t num_rows, int64_t x_len, int vals_dtype);
+
 /* CSR SpMV y = A x: y[r] = sum over row r of vals[p] * x[cols[p]], rows
  * delimited as in tcs_seg_sum_indptr with ends = indptr + 1 (num_rows
  * entries).  vals/x: F16 or F32 (same dtype); products are formed exactly
- * and summed in double, y is float32.  Columns outside [0, x_len)
- * contribute 0.  Uses tcs_seg_sum_workspace_bytes(nnz).
+ * and summed in double, y is float32 (the correctly rounded row sum while
+ * the exact sum fits in double).  Columns outside [0, x_len) contribute
+ * exactly 0.  Workspace: tcs_spmv_workspace_bytes (or at least
+ * tcs_seg_sum_workspace_bytes(nnz)).
  * Replaces: spmv_v2_{fp16,fp32} (src/spmv_v2.cpp:126,162) behind
  * run_spmv_v2 (src/torch/torch_spmv.h:112). */
 int tcs_spmv_csr(const void* vals, int vals_dtype, const void* indptr_ends, int ends_dtype,
@@ -268,11 +277,27 @@ int tcs_csr_gather(const void* vals, int dtype, const int32_t* cols, const void*
 
 size_t tcs_compress_workspace_bytes(int64_t n);
 
-/* *count_out (device int64) = number of i with mask[i] == 1.
+/* *count_out (device int64) = number of i with mask[i] == 1, n < 2^31.
+ * Also leaves in the workspace the count of every 2048-element chunk and
+ * the offset of every 64K-element block, which tcs_compress_counted reads.
  * Replaces the sizing step of run_compress / run_compress_ind /
  * _no_arange (run_reduce_tiles + .item(), src/torch/torch_compress.h:62-64). */
 int tcs_count_mask(const int8_t* mask, int64_t n, int64_t* count_out, void* workspace,
                    size_t workspace_bytes, void* stream);
+
+/* tcs_compress for a mask that tcs_count_mask(mask, n) counted last on the
+ * same workspace (stream-ordered, nothing else on that workspace between
+ * the two calls): output offsets come from the counts it left, so every
+ * 2048-element chunk is compacted independently, without the look-back.
+ * Same arguments and results as tcs_compress.
+ * Replaces: compress_fp16/_fp32, compress_ind_*, compress_ind_no_arange_*
+ * (src/compress.cpp:26-242) behind run_compress, run_compress_ind and
+ * run_compress_ind_no_arange (src/torch/torch_compress.h:41,172,249),
+ * which use the counts of their own sizing pass the same way
+ * (num_ones_per_block, src/torch/torch_compress.h:196-197). */
+int tcs_compress_counted(const void* x, int x_elem_bytes, const void* idx_in, int idx_mode, int idx_elem_bytes,
+                         const int8_t* mask, void* out_x, void* out_idx, int64_t n, int64_t out_capacity,
+                         void* workspace, size_t workspace_bytes, void* stream);
 
 /* Stable compaction of the elements with mask[i] == 1 into out_x (element
  * size x_elem_bytes: 1, 2, 4 or 8; bits are copied).  idx_mode 0: values
@@ -456,6 +481,20 @@ size_t tcs_batch_workspace_bytes(const tcs_batch* b);
  * for the process; `force` re-tunes cached keys.  Synchronises `stream`. */
 int tcs_batch_tune(tcs_batch* b, void* workspace, size_t workspace_bytes, int reps, int force,
                    void* stream);
+
+/* The tuner's cache of this process, to persist it across processes:
+ * entry i (0 <= i < tcs_batch_tuned_count(), in key order) is the best
+ * configuration found for scans of kind `kind` (0 = plain scan, 1 =
+ * flag-segmented scan), dtype `dtype` and ceil(log2 n) = log2n.
+ * tcs_batch_tuned_set adds or replaces one (validated as a candidate); ops
+ * added to a plan afterwards, and tcs_batch_tune without `force`, use it.
+ * No counterpart in the reference, whose tiling comes from fixed host
+ * heuristics (src/tiling/heuristics/). */
+int tcs_batch_tuned_count(void);
+int tcs_batch_tuned_entry(int i, int* kind, int* dtype, int* log2n, int* S, int* tiles_per_wg, int* method,
+                          int* window, float* ms);
+int tcs_batch_tuned_set(int kind, int dtype, int log2n, int S, int tiles_per_wg, int method, int window,
+                        float ms);
 
 /* Configuration of op `op` (S, tiles_per_wg, method, window; ms = the
  * tuner's best time, -1 if untuned). */

@@ -15,6 +15,7 @@
 // one captured graph correct.
 #include <algorithm>
 #include <cmath>
+#include <iterator>
 #include <map>
 #include <mutex>
 #include <tuple>
@@ -158,8 +159,10 @@ int add_op(tcs_batch* b, Op op) {
       need = tcs_scan_workspace_bytes(op.n, 32, 1);
       break;
     case kOpSegSum:
-    case kOpSpmv:
       need = tcs_seg_sum_workspace_bytes(op.n);
+      break;
+    case kOpSpmv:
+      need = tcs_spmv_workspace_bytes(op.n, op.rows, op.xv_len, op.x_dtype);
       break;
   }
   size_t off = 0;
@@ -331,6 +334,50 @@ int tcs_batch_tune(tcs_batch* b, void* ws, size_t ws_bytes, int reps, int force,
   (void)hipEventDestroy(e1);
   if (!rc) b->drop_graph();
   return rc;
+}
+
+int tcs_batch_tuned_count(void) {
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  return (int)g_tuned.size();
+}
+
+int tcs_batch_tuned_entry(int i, int* kind, int* dtype, int* log2n, int* S, int* tiles_per_wg, int* method,
+                          int* window, float* ms) {
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  if (i < 0 || i >= (int)g_tuned.size()) return TCS_ERR_INVALID_ARG;
+  auto it = g_tuned.begin();
+  std::advance(it, i);
+  if (kind) *kind = std::get<0>(it->first);
+  if (dtype) *dtype = std::get<1>(it->first);
+  if (log2n) *log2n = std::get<2>(it->first);
+  if (S) *S = it->second.S;
+  if (tiles_per_wg) *tiles_per_wg = it->second.tiles_per_wg;
+  if (method) *method = it->second.method;
+  if (window) *window = it->second.window;
+  if (ms) *ms = it->second.ms;
+  return TCS_OK;
+}
+
+int tcs_batch_tuned_set(int kind, int dtype, int log2n, int S, int tiles_per_wg, int method, int window,
+                        float ms) {
+  if ((kind != kOpScan && kind != kOpSegScan) || log2n < 0 || log2n > 62) return TCS_ERR_INVALID_ARG;
+  if (dtype != TCS_F16 && dtype != TCS_BF16 && dtype != TCS_F32 && dtype != TCS_I8 && dtype != TCS_I32)
+    return TCS_ERR_UNSUPPORTED_DTYPE;
+  if ((S != 32 && S != 64 && S != 128) || tiles_per_wg < 1 || tiles_per_wg > 32 ||
+      (tiles_per_wg & (tiles_per_wg - 1)) != 0 || window < 1 || window > tcs::kWave)
+    return TCS_ERR_INVALID_ARG;
+  if (method != TCS_METHOD_DPP && method != TCS_METHOD_MFMA) return TCS_ERR_INVALID_ARG;
+  if (method == TCS_METHOD_MFMA && (kind != kOpScan || (dtype != TCS_F16 && dtype != TCS_BF16 && dtype != TCS_I8)))
+    return TCS_ERR_UNSUPPORTED_DTYPE;
+  ScanCfg c;
+  c.S = S;
+  c.tiles_per_wg = tiles_per_wg;
+  c.method = method;
+  c.window = window;
+  c.ms = ms;
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  g_tuned[std::make_tuple(kind, dtype, log2n)] = c;
+  return TCS_OK;
 }
 
 int tcs_batch_op_config(const tcs_batch* b, int op, int* S, int* tiles_per_wg, int* method, int* window,

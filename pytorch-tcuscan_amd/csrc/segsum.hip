@@ -22,6 +22,20 @@
 // correctly rounded to float; SpMV products are formed exactly), int8 in
 // int32 (exact).  The reference rounds SpMV products to fp16
 // (src/kernels/kernel_csr_gather.h:131-133); this build does not.
+//
+// SpMV over a vector larger than about half an L2 splits the gather by
+// column range across the XCDs.  MI355X dispatches workgroup b to XCD
+// b mod 8 and every XCD has its own 4 MB L2, so with two workgroups per
+// tile, the even one summing the products whose column lies in the lower
+// half of x and the odd one the upper half, each L2 caches only its half
+// of x (2^22 fp16 columns: random gathers 0.61 -> 0.41 ms).  The two
+// halves are two independent segmented reductions, each with its own
+// look-back chain, writing exact double partial sums per row; a combine
+// pass adds the two partials of every row and rounds once to float, so the
+// result is bit-identical to the single-chain sum.  Correctness never
+// depends on where workgroups run, only the cache hit rate does.
+#include <type_traits>
+
 #include "tcs_common.hpp"
 
 namespace tcs {
@@ -39,10 +53,12 @@ __device__ __forceinline__ int64_t row_end(const IdxT* __restrict__ ends, int64_
 
 // Row pass: zero empty rows, record each tile's first row.  tile_row[b] = R
 // marks a tile past the last row.
+// With `part` set (the split SpMV) empty rows zero both partial sums
+// part[r] and part[R + r] instead of y[r].
 template <class IdxT, class OutT>
 __global__ __launch_bounds__(256) void seg_rows_kernel(const IdxT* __restrict__ ends, int64_t R,
                                                        int64_t n_ends, int64_t nnz,
-                                                       OutT* __restrict__ y,
+                                                       OutT* __restrict__ y, double* __restrict__ part,
                                                        uint64_t* __restrict__ tile_row,
                                                        uint32_t num_tiles) {
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -50,7 +66,12 @@ __global__ __launch_bounds__(256) void seg_rows_kernel(const IdxT* __restrict__ 
   const int64_t s = row_end(ends, r - 1, n_ends, nnz);
   const int64_t e = row_end(ends, r, n_ends, nnz);
   if (e <= s) {
-    y[r] = OutT(0);
+    if (part) {
+      part[r] = 0.0;
+      part[R + r] = 0.0;
+    } else {
+      y[r] = OutT(0);
+    }
   } else {
     for (int64_t b = (s + kSegTile - 1) / kSegTile; b * kSegTile < e && b < num_tiles; ++b)
       tile_row[b] = (uint64_t)r;
@@ -72,16 +93,23 @@ struct SrcPlain {
   __device__ __forceinline__ void load(Regs& r, int j, int64_t gb, int64_t t1) const {
     load_items<In, kSegG, FULL>(x, gb, t1, r.v[j]);
   }
+  template <bool FULL>
+  __device__ __forceinline__ void gather(Regs&, int, int64_t, int64_t) const {}
   __device__ __forceinline__ Acc value(const Regs& r, int j, int i) const { return C::conv(r.v[j][i]); }
   __device__ __forceinline__ Acc value_at(int64_t p) const { return C::conv(x[p]); }
   __device__ __forceinline__ void barrier(Regs& r) const {
 #pragma unroll
     for (int j = 0; j < kSegJ; ++j) reg_barrier(r.v[j]);
   }
+  __device__ __forceinline__ void set_part(int, int) {}
+  static constexpr bool kSplittable = false;
 };
 
-// SpMV products vals[p] * xv[cols[p]] formed exactly in double.  Columns
-// outside [0, x_len) contribute 0.
+// SpMV products vals[p] * xv[cols[p]], formed exactly: an fp16 product has
+// at most 22 significant bits, so float holds it exactly; fp32 products are
+// formed in double.  Only columns in [c_lo, c_lo + c_n) contribute (all of
+// [0, x_len) unless set_part narrows it to one part of a split), others
+// and columns outside [0, x_len) contribute exactly 0.
 template <class C>
 struct SrcSpmv {
   using Acc = double;
@@ -90,41 +118,86 @@ struct SrcSpmv {
   const int32_t* __restrict__ cols;
   const In* __restrict__ xv;
   int64_t x_len;
-  struct Regs { In v[kSegJ][kSegG]; In g[kSegJ][kSegG]; };
+  uint32_t c_lo = 0, c_n = 0;
+  static constexpr bool kSplittable = true;
+  struct Regs {
+    Packed<In, kSegG> v[kSegJ];
+    Packed<In, kSegG> g[kSegJ];
+    uint32_t c[kSegJ][kSegG];  // column indices, from load() until gather()
+    uint32_t ok[kSegJ];        // bit i: item i of group j contributes
+  };
+  // Part `part` of `parts` equal column ranges (the last one shorter).
+  __device__ __forceinline__ void set_part(int part, int parts) {
+    // Columns are int32: nothing at or past 2^31 is ever gathered.
+    const uint64_t len = x_len < ((int64_t)1 << 31) ? (uint64_t)x_len : ((uint64_t)1 << 31);
+    const uint64_t w = (len + (uint64_t)parts - 1) / (uint64_t)parts;
+    const uint64_t lo = w * (uint64_t)part < len ? w * (uint64_t)part : len;
+    const uint64_t hi = lo + w < len ? lo + w : len;
+    c_lo = (uint32_t)lo;
+    c_n = (uint32_t)(hi - lo);
+  }
+  __device__ __forceinline__ bool in_range(int32_t c) const { return (uint32_t)c - c_lo < c_n; }
   // Values and column indices stream through once: nontemporal loads on
   // whole groups, so that they do not displace the gathered vector x from
   // the caches.
-  template <class T>
-  __device__ __forceinline__ static void load_nt(const T* __restrict__ p, int64_t base, T (&r)[kSegG]) {
-    constexpr int kBytes = kSegG * (int)sizeof(T);
-    static_assert(kBytes % 16 == 0, "whole 16-byte vectors");
+  template <class T, int W>
+  __device__ __forceinline__ static void load_nt(const T* __restrict__ p, int64_t base, uint32_t (&w)[W]) {
+    static_assert(W * 4 == kSegG * (int)sizeof(T) && W % 4 == 0, "whole 16-byte vectors");
     const u32x4* vp = reinterpret_cast<const u32x4*>(p + base);
-    u32x4 tmp[kBytes / 16];
 #pragma unroll
-    for (int k = 0; k < kBytes / 16; ++k) tmp[k] = __builtin_nontemporal_load(vp + k);
-    __builtin_memcpy(r, tmp, kBytes);
+    for (int k = 0; k < W / 4; ++k) {
+      const u32x4 t = __builtin_nontemporal_load(vp + k);
+      w[4 * k] = t[0];
+      w[4 * k + 1] = t[1];
+      w[4 * k + 2] = t[2];
+      w[4 * k + 3] = t[3];
+    }
   }
+  // The streamed loads of group j; gather() then fetches x for them (the
+  // caller issues other loads in between, so that the column indices'
+  // latency is not waited out twice).
   template <bool FULL>
   __device__ __forceinline__ void load(Regs& r, int j, int64_t gb, int64_t t1) const {
-    int32_t c[kSegG];
     if constexpr (FULL) {
-      load_nt<In>(vals, gb, r.v[j]);
-      load_nt<int32_t>(cols, gb, c);
+      load_nt<In>(vals, gb, r.v[j].w);
+      load_nt<int32_t>(cols, gb, r.c[j]);
     } else {
       load_items<In, kSegG, false>(vals, gb, t1, r.v[j]);
+      int32_t c[kSegG];
       load_items<int32_t, kSegG, false>(cols, gb, t1, c);
+      __builtin_memcpy(r.c[j], c, sizeof(c));
     }
+  }
+  template <bool FULL>
+  __device__ __forceinline__ void gather(Regs& r, int j, int64_t gb, int64_t t1) const {
+    In g[kSegG];
+    uint32_t ok = 0;
 #pragma unroll
-    for (int i = 0; i < kSegG; ++i)
-      r.g[j][i] = ((uint32_t)c[i] < (uint64_t)x_len && (FULL || gb + i < t1)) ? xv[c[i]] : In(0);
+    for (int i = 0; i < kSegG; ++i) {
+      const bool k = in_range((int32_t)r.c[j][i]) && (FULL || gb + i < t1);
+      // A 32-bit byte offset from a uniform base (x_len < 2^31): one VGPR
+      // per address instead of a 64-bit pointer.
+      const uint32_t off = r.c[j][i] * (uint32_t)sizeof(In);
+      g[i] = k ? *reinterpret_cast<const In*>(reinterpret_cast<const char*>(xv) + off) : In(0);
+      ok |= (k ? 1u : 0u) << i;
+    }
+    __builtin_memcpy(r.g[j].w, g, sizeof(g));
+    r.ok[j] = ok;
+  }
+  __device__ __forceinline__ static double prod(In a, In b) {
+    if constexpr (sizeof(In) == 2) {
+      return (double)((float)C::conv(a) * (float)C::conv(b));
+    } else {
+      return (double)C::conv(a) * (double)C::conv(b);
+    }
   }
   __device__ __forceinline__ Acc value(const Regs& r, int j, int i) const {
-    return (double)C::conv(r.v[j][i]) * (double)C::conv(r.g[j][i]);
+    return (r.ok[j] >> i & 1u) ? prod(r.v[j][i], r.g[j][i]) : 0.0;
   }
   __device__ __forceinline__ Acc value_at(int64_t p) const {
     const int32_t c = cols[p];
-    if ((uint32_t)c >= (uint64_t)x_len) return 0.0;
-    return (double)C::conv(vals[p]) * (double)C::conv(xv[c]);
+    if (!in_range(c)) return 0.0;
+    return prod(vals[p], xv[c]);
   }
   __device__ __forceinline__ void barrier(Regs& r) const {
 #pragma unroll
@@ -135,10 +208,15 @@ struct SrcSpmv {
   }
 };
 
-template <class Src, class IdxT, class OutT>
+// SPLIT workgroups per tile: workgroup b reduces tile b / SPLIT over part
+// b % SPLIT of the values (see the split SpMV above), with that part's own
+// descriptors, and writes its row sums as doubles to part[p * R + row];
+// SPLIT = 1 writes y.
+template <class Src, class IdxT, class OutT, int SPLIT>
 __global__ __launch_bounds__(kSegNW * kWave) void seg_reduce_kernel(
     Src src, const IdxT* __restrict__ ends, int64_t R, int64_t n_ends, int64_t nnz,
-    OutT* __restrict__ y, const uint64_t* __restrict__ tile_row, uint32_t num_tiles, void* ws) {
+    OutT* __restrict__ y, double* __restrict__ part_out, const uint64_t* __restrict__ tile_row,
+    uint32_t num_tiles, void* ws) {
   using Acc = typename Src::Acc;
   constexpr int G = kSegG, J = kSegJ, NW = kSegNW;
   constexpr int64_t GSPAN = (int64_t)kWave * G, WSEG = GSPAN * J;
@@ -153,30 +231,50 @@ __global__ __launch_bounds__(kSegNW * kWave) void seg_reduce_kernel(
   if (threadIdx.x == 0) s_tag = launch_arrive(ws);
   const int lane = __lane_id();
   const int wid = threadIdx.x / kWave;
-  const uint32_t tile = blockIdx.x;
+  const uint32_t tile = blockIdx.x / SPLIT;
+  const int part = SPLIT > 1 ? (int)(blockIdx.x % SPLIT) : 0;
+  src.set_part(part, SPLIT);
   const int64_t t0 = (int64_t)tile * kSegTile;
+  // Loads that do not depend on one another go out together: the last row
+  // end, the tile's first rows, and the tile's values (bounded by nnz;
+  // values past the last row end belong to no row and are never stored).
   const int64_t end_all = row_end(ends, R - 1, n_ends, nnz);
-  if (t0 >= end_all) return;  // no row reaches this tile (nor any later one)
-  const int64_t t1 = t0 + kSegTile < end_all ? t0 + kSegTile : end_all;
-
-  // Rows touching the tile: r_lo (holds t0) .. r_hi; their ends are
-  // end(r_lo - 1 + k) for k in [0, cnt).
   int64_t r_lo = (int64_t)tile_row[tile];
   int64_t r_hi = tile + 1 < num_tiles ? (int64_t)tile_row[tile + 1] : R - 1;
-  if (r_lo > R - 1) r_lo = R - 1;
-  if (r_hi > R - 1) r_hi = R - 1;
-  if (r_hi < r_lo) r_hi = r_lo;
-  const int64_t cnt = r_hi - r_lo + 2;
-
+  const int64_t t1 = t0 + kSegTile < nnz ? t0 + kSegTile : nnz;
   typename Src::Regs regs;
   const int64_t wbase = t0 + (int64_t)wid * WSEG + (int64_t)lane * G;
-  const bool full = t0 + kSegTile <= t1;
+  const bool full = t0 + kSegTile <= nnz;
   if (full) {
 #pragma unroll
     for (int j = 0; j < J; ++j) src.template load<true>(regs, j, wbase + j * GSPAN, t1);
   } else {
 #pragma unroll
     for (int j = 0; j < J; ++j) src.template load<false>(regs, j, wbase + j * GSPAN, t1);
+  }
+  if (t0 >= end_all) return;  // no row reaches this tile (nor any later one)
+
+  // Rows touching the tile: r_lo (holds t0) .. r_hi; their ends are
+  // end(r_lo - 1 + k) for k in [0, cnt).  The ends of the first
+  // 2 * NW * 64 are loaded before the gathers are issued.
+  if (r_lo > R - 1) r_lo = R - 1;
+  if (r_hi > R - 1) r_hi = R - 1;
+  if (r_hi < r_lo) r_hi = r_lo;
+  const int64_t cnt = r_hi - r_lo + 2;
+  constexpr int kEarly = 2;
+  int64_t e_early[kEarly], ep_early[kEarly];
+#pragma unroll
+  for (int q = 0; q < kEarly; ++q) {
+    const int64_t k = threadIdx.x + (int64_t)q * NW * kWave;
+    e_early[q] = k < cnt ? row_end(ends, r_lo - 1 + k, n_ends, nnz) : -1;
+    ep_early[q] = (k > 0 && k < cnt) ? row_end(ends, r_lo - 2 + k, n_ends, nnz) : -1;
+  }
+  if (full) {
+#pragma unroll
+    for (int j = 0; j < J; ++j) src.template gather<true>(regs, j, wbase + j * GSPAN, t1);
+  } else {
+#pragma unroll
+    for (int j = 0; j < J; ++j) src.template gather<false>(regs, j, wbase + j * GSPAN, t1);
   }
 
   // Mark the boundaries.  Among equal ends (empty rows) only the first, the
@@ -187,10 +285,8 @@ __global__ __launch_bounds__(kSegNW * kWave) void seg_reduce_kernel(
     s_endb[k] = 0;
   }
   __syncthreads();
-  for (int64_t k = threadIdx.x; k < cnt; k += NW * kWave) {
-    const int64_t e = row_end(ends, r_lo - 1 + k, n_ends, nnz);
-    const int64_t ep = k > 0 ? row_end(ends, r_lo - 2 + k, n_ends, nnz) : -1;
-    if (k > 0 && ep == e) continue;  // an empty row: the same boundary as before
+  auto mark = [&](int64_t k, int64_t e, int64_t ep) {
+    if (k > 0 && ep == e) return;  // an empty row: the same boundary as before
     if (e >= t0 && e < t1) {
       const int q = (int)(e - t0);
       atomicOr(&s_head[q >> 5], 1u << (q & 31));
@@ -200,7 +296,14 @@ __global__ __launch_bounds__(kSegNW * kWave) void seg_reduce_kernel(
       atomicOr(&s_endb[q >> 5], 1u << (q & 31));
       s_rowk[q] = (uint32_t)k;
     }
+  };
+#pragma unroll
+  for (int q = 0; q < kEarly; ++q) {
+    const int64_t k = threadIdx.x + (int64_t)q * NW * kWave;
+    if (k < cnt) mark(k, e_early[q], ep_early[q]);
   }
+  for (int64_t k = threadIdx.x + (int64_t)kEarly * NW * kWave; k < cnt; k += NW * kWave)
+    mark(k, row_end(ends, r_lo - 1 + k, n_ends, nnz), row_end(ends, r_lo - 2 + k, n_ends, nnz));
   __syncthreads();
 
   // Heads and ends of each lane group (G = 8 elements at an 8-aligned offset).
@@ -278,7 +381,7 @@ __global__ __launch_bounds__(kSegNW * kWave) void seg_reduce_kernel(
       for (int64_t p = from + lane; p < u1; p += kWave) sum = sum + src.value_at(p);
       return FlagVal<Acc>{s >= u0 ? 1 : 0, wave_sum(sum)};
     };
-    uint64_t* desc = ws_desc(ws);
+    uint64_t* desc = ws_desc(ws) + (size_t)part * num_tiles * (kDescBytes / 8);
     const uint32_t tag = s_tag.tag;
     const int max_polls = s_tag.polls;
     Acc carry = Acc(0);
@@ -319,42 +422,91 @@ __global__ __launch_bounds__(kSegNW * kWave) void seg_reduce_kernel(
         int64_t row = r_lo - 1 + (int64_t)s_rowk[q0 + i];
         if (row < 0) row = 0;
         if (row > R - 1) row = R - 1;
-        y[row] = (OutT)run;
+        if constexpr (SPLIT > 1) {
+          part_out[(int64_t)part * R + row] = (double)run;
+        } else {
+          y[row] = (OutT)run;
+        }
       }
     }
   }
 }
 
+// y[r] = float(part[r] + part[R + r]) for every row, and the partials
+// zeroed again: the workspace carries no doubles from one call to the next
+// (a later call may read these bytes as look-back descriptors).
+__global__ __launch_bounds__(256) void spmv_combine_kernel(double* __restrict__ part, int64_t R,
+                                                           float* __restrict__ y) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= R) return;
+  const double a = part[r], b = part[R + r];
+  y[r] = (float)(a + b);
+  part[r] = 0.0;
+  part[R + r] = 0.0;
+}
+
+// Workspace of one call: the look-back region (header, `parts` descriptor
+// chains of num_tiles tiles), the first row of every tile, and with
+// parts > 1 the two partial sums of every row.
 struct SegSumPlan {
   uint32_t num_tiles;
+  int parts;
   size_t ws_bytes;
   size_t tile_row_off;
+  size_t part_off;
 };
 
-inline SegSumPlan seg_plan(int64_t nnz) {
+inline SegSumPlan seg_plan(int64_t nnz, int parts = 1, int64_t rows = 0) {
   SegSumPlan p;
   p.num_tiles = (uint32_t)((nnz + kSegTile - 1) / kSegTile);
-  const size_t lb = lookback_ws_bytes(p.num_tiles);
+  p.parts = parts;
+  const size_t lb = lookback_ws_bytes((uint64_t)p.num_tiles * (uint64_t)parts);
   p.tile_row_off = lb;
-  p.ws_bytes = lb + (((size_t)p.num_tiles * 8 + 255) & ~(size_t)255);
+  p.part_off = lb + (((size_t)p.num_tiles * 8 + 255) & ~(size_t)255);
+  p.ws_bytes = p.part_off + (parts > 1 ? (((size_t)rows * 16 + 255) & ~(size_t)255) : 0);
   return p;
+}
+
+// Vectors of more than this many bytes take the split SpMV: each L2 (4 MB)
+// then caches half of x.
+constexpr size_t kSpmvSplitMinXBytes = (size_t)2 << 20;
+
+inline int spmv_parts(int64_t x_len, int vals_dtype) {
+  const size_t eb = vals_dtype == TCS_F32 ? 4 : 2;
+  return (size_t)x_len * eb > kSpmvSplitMinXBytes ? 2 : 1;
 }
 
 template <class Src, class IdxT, class OutT>
 static int launch_seg(const Src& src, const void* ends_v, int64_t R, int64_t n_ends, int64_t nnz,
-                      void* y_v, void* ws, size_t ws_bytes, hipStream_t st) {
-  const SegSumPlan plan = seg_plan(nnz);
+                      void* y_v, void* ws, size_t ws_bytes, hipStream_t st, int parts = 1) {
+  SegSumPlan plan = seg_plan(nnz, parts, R);
+  // One chain when there is nothing to gather or no room for the split.
+  if (parts > 1 && (nnz == 0 || !ws || ws_bytes < plan.ws_bytes)) plan = seg_plan(nnz);
   if (!ws || ws_bytes < plan.ws_bytes) return TCS_ERR_WORKSPACE;
   const auto* ends = static_cast<const IdxT*>(ends_v);
   auto* y = static_cast<OutT*>(y_v);
   auto* tile_row = reinterpret_cast<uint64_t*>(static_cast<char*>(ws) + plan.tile_row_off);
+  double* part = plan.parts > 1 ? reinterpret_cast<double*>(static_cast<char*>(ws) + plan.part_off) : nullptr;
   const int64_t row_blocks = (R + 255) / 256;
   hipLaunchKernelGGL((seg_rows_kernel<IdxT, OutT>), dim3((uint32_t)row_blocks), dim3(256), 0, st,
-                     ends, R, n_ends, nnz, y, tile_row, plan.num_tiles);
+                     ends, R, n_ends, nnz, y, part, tile_row, plan.num_tiles);
   TCS_CHECK_LAUNCH();
   if (plan.num_tiles == 0) return TCS_OK;
-  hipLaunchKernelGGL((seg_reduce_kernel<Src, IdxT, OutT>), dim3(plan.num_tiles), dim3(kSegNW * kWave),
-                     0, st, src, ends, R, n_ends, nnz, y, tile_row, plan.num_tiles, ws);
+  if (plan.parts > 1) {
+    if constexpr (Src::kSplittable && std::is_same<OutT, float>::value) {
+      hipLaunchKernelGGL((seg_reduce_kernel<Src, IdxT, OutT, 2>), dim3(plan.num_tiles * 2u),
+                         dim3(kSegNW * kWave), 0, st, src, ends, R, n_ends, nnz, y, part, tile_row,
+                         plan.num_tiles, ws);
+      TCS_CHECK_LAUNCH();
+      hipLaunchKernelGGL(spmv_combine_kernel, dim3((uint32_t)row_blocks), dim3(256), 0, st, part, R, y);
+      TCS_CHECK_LAUNCH();
+      return TCS_OK;
+    } else {
+      return TCS_ERR_UNSUPPORTED_DTYPE;
+    }
+  }
+  hipLaunchKernelGGL((seg_reduce_kernel<Src, IdxT, OutT, 1>), dim3(plan.num_tiles), dim3(kSegNW * kWave),
+                     0, st, src, ends, R, n_ends, nnz, y, nullptr, tile_row, plan.num_tiles, ws);
   TCS_CHECK_LAUNCH();
   return TCS_OK;
 }
@@ -362,13 +514,13 @@ static int launch_seg(const Src& src, const void* ends_v, int64_t R, int64_t n_e
 template <class Src, class OutT>
 static int launch_seg_idx(const Src& src, const void* indptr, int idx_dtype, int64_t R,
                           int64_t n_ends, int64_t nnz, void* y, void* ws, size_t ws_bytes,
-                          hipStream_t st) {
+                          hipStream_t st, int parts = 1) {
   if (idx_dtype == TCS_I32)
-    return launch_seg<Src, int32_t, OutT>(src, indptr, R, n_ends, nnz, y, ws, ws_bytes, st);
+    return launch_seg<Src, int32_t, OutT>(src, indptr, R, n_ends, nnz, y, ws, ws_bytes, st, parts);
   if (idx_dtype == TCS_U32)
-    return launch_seg<Src, uint32_t, OutT>(src, indptr, R, n_ends, nnz, y, ws, ws_bytes, st);
+    return launch_seg<Src, uint32_t, OutT>(src, indptr, R, n_ends, nnz, y, ws, ws_bytes, st, parts);
   if (idx_dtype == TCS_I64)
-    return launch_seg<Src, int64_t, OutT>(src, indptr, R, n_ends, nnz, y, ws, ws_bytes, st);
+    return launch_seg<Src, int64_t, OutT>(src, indptr, R, n_ends, nnz, y, ws, ws_bytes, st, parts);
   return TCS_ERR_UNSUPPORTED_DTYPE;
 }
 
@@ -378,6 +530,13 @@ extern "C" {
 
 size_t tcs_seg_sum_workspace_bytes(int64_t nnz) {
   return tcs::seg_plan(nnz < 0 ? 0 : nnz).ws_bytes;
+}
+
+size_t tcs_spmv_workspace_bytes(int64_t nnz, int64_t num_rows, int64_t x_len, int vals_dtype) {
+  if (nnz < 0) nnz = 0;
+  if (num_rows < 0) num_rows = 0;
+  if (x_len < 0) x_len = 0;
+  return tcs::seg_plan(nnz, tcs::spmv_parts(x_len, vals_dtype), num_rows).ws_bytes;
 }
 
 int tcs_seg_sum_indptr(const void* x, int x_dtype, const void* ends, int ends_dtype,
@@ -427,16 +586,17 @@ int tcs_spmv_csr(const void* vals, int vals_dtype, const void* indptr_ends, int 
   if (ends_dtype != TCS_I32 && ends_dtype != TCS_U32 && ends_dtype != TCS_I64)
     return TCS_ERR_UNSUPPORTED_DTYPE;
   hipStream_t st = static_cast<hipStream_t>(stream);
+  const int parts = spmv_parts(x_len, vals_dtype);
   switch (vals_dtype) {
     case TCS_F16: {
       SrcSpmv<CodecF16> src{static_cast<const uint16_t*>(vals), cols, static_cast<const uint16_t*>(x), x_len};
       return launch_seg_idx<SrcSpmv<CodecF16>, float>(src, indptr_ends, ends_dtype, num_rows, num_rows, nnz,
-                                                      y, workspace, workspace_bytes, st);
+                                                      y, workspace, workspace_bytes, st, parts);
     }
     case TCS_F32: {
       SrcSpmv<CodecF32> src{static_cast<const float*>(vals), cols, static_cast<const float*>(x), x_len};
       return launch_seg_idx<SrcSpmv<CodecF32>, float>(src, indptr_ends, ends_dtype, num_rows, num_rows, nnz,
-                                                      y, workspace, workspace_bytes, st);
+                                                      y, workspace, workspace_bytes, st, parts);
     }
     default:
       return TCS_ERR_UNSUPPORTED_DTYPE;

@@ -169,7 +169,7 @@ class Server {
     e.rows = indptr.numel() - 1;
     e.out_elems = e.rows;
     e.out_bytes = (size_t)e.rows * 4;
-    e.ws_bytes = tcs_seg_sum_workspace_bytes(e.n);
+    e.ws_bytes = tcs_spmv_workspace_bytes(e.n, e.rows, xv.numel(), dcode(vals.scalar_type()));
     e.bytes_moved = (double)e.n * (vals.element_size() + 4) + 8.0 * e.rows + (double)xv.numel() * xv.element_size();
     return add(std::move(e));
   }
@@ -431,8 +431,8 @@ class Server {
         int64_t* cnt = reinterpret_cast<int64_t*>(out + up256((size_t)e.n * e.a.element_size()));
         rc = tcs_count_mask(static_cast<const int8_t*>(e.b.data_ptr()), e.n, cnt, ws, wb, st);
         if (rc == TCS_OK)
-          rc = tcs_compress(e.a.data_ptr(), (int)e.a.element_size(), nullptr, 0, 4,
-                            static_cast<const int8_t*>(e.b.data_ptr()), out, nullptr, e.n, e.n, ws, wb, st);
+          rc = tcs_compress_counted(e.a.data_ptr(), (int)e.a.element_size(), nullptr, 0, 4,
+                                    static_cast<const int8_t*>(e.b.data_ptr()), out, nullptr, e.n, e.n, ws, wb, st);
         if (rc == TCS_OK)
           hip_ok(hipMemcpyAsync(pinned_ + slot, cnt, sizeof(int64_t), hipMemcpyDeviceToHost, st), "count copy");
         break;

@@ -78,10 +78,17 @@ struct Workspace {
 std::mutex g_ws_mu;
 std::map<std::pair<int, void*>, Workspace> g_ws;
 
-Workspace& workspace_entry(const at::Tensor& like, size_t bytes) {
+// keep = true fetches the workspace for a call that reads what the previous
+// call on it left there (a counted compaction after its count pass): it is
+// neither replaced nor re-zeroed, and the call is not counted.
+Workspace& workspace_entry(const at::Tensor& like, size_t bytes, bool keep = false) {
   const int dev = like.device().index();
   void* s = stream_of(like);
   auto& w = g_ws[{dev, s}];
+  if (keep) {
+    TORCH_CHECK(w.t.defined() && (size_t)w.t.numel() >= bytes, "workspace: the count pass left no workspace");
+    return w;
+  }
   bool fresh = false;
   if (!w.t.defined() || (size_t)w.t.numel() < bytes) {
     size_t want = bytes < (1u << 20) ? (1u << 20) : bytes;
@@ -102,9 +109,9 @@ Workspace& workspace_entry(const at::Tensor& like, size_t bytes) {
   return w;
 }
 
-std::pair<void*, size_t> workspace(const at::Tensor& like, size_t bytes) {
+std::pair<void*, size_t> workspace(const at::Tensor& like, size_t bytes, bool keep = false) {
   std::lock_guard<std::mutex> lock(g_ws_mu);
-  Workspace& w = workspace_entry(like, bytes);
+  Workspace& w = workspace_entry(like, bytes, keep);
   return {w.t.data_ptr(), (size_t)w.t.numel()};
 }
 
@@ -543,7 +550,7 @@ at::Tensor run_spmv_v2(const at::Tensor& vals_in, const at::Tensor& indptr_in,
   TORCH_CHECK(R >= 0, op, ": indptr must not be empty");
   at::Tensor z = at::empty({R}, vals.options().dtype(at::kFloat));
   if (R == 0) return z;
-  auto ws = workspace(vals, tcs_seg_sum_workspace_bytes(vals.numel()));
+  auto ws = workspace(vals, tcs_spmv_workspace_bytes(vals.numel(), R, x.numel(), dtype_code(vt)));
   const void* ends = static_cast<const char*>(indptr.data_ptr()) + indptr.element_size();
   const at::hip::HIPGuardMasqueradingAsCUDA guard(vals.device());
   check(tcs_spmv_csr(vals.numel() ? vals.data_ptr() : nullptr, dtype_code(vt), ends, index_code(indptr),
@@ -659,17 +666,20 @@ int64_t count_ones(const at::Tensor& mask, const char* op) {
   return cnt.item<int64_t>();
 }
 
+// counted = true: count_ones(mask) ran last on this stream's workspace, so
+// the compaction takes its chunk offsets from there (tcs_compress_counted);
+// otherwise it finds them by look-back (tcs_compress).
 void compress_into(const at::Tensor& x, const at::Tensor* idx_in, int idx_mode, const at::Tensor& mask,
-                   at::Tensor& out_x, at::Tensor* out_idx, const char* op) {
+                   at::Tensor& out_x, at::Tensor* out_idx, const char* op, bool counted) {
   const int64_t n = x.numel();
   if (n == 0) return;
-  auto ws = workspace(x, tcs_compress_workspace_bytes(n));
+  auto ws = workspace(x, tcs_compress_workspace_bytes(n), counted);
   const at::hip::HIPGuardMasqueradingAsCUDA guard(x.device());
-  check(tcs_compress(x.data_ptr(), (int)x.element_size(), idx_in ? idx_in->data_ptr() : nullptr, idx_mode,
-                     idx_in ? (int)idx_in->element_size() : 4, static_cast<const int8_t*>(mask.data_ptr()),
-                     out_x.numel() ? out_x.data_ptr() : nullptr,
-                     (out_idx && out_idx->numel()) ? out_idx->data_ptr() : nullptr, n, out_x.numel(),
-                     ws.first, ws.second, stream_of(x)),
+  auto fn = counted ? tcs_compress_counted : tcs_compress;
+  check(fn(x.data_ptr(), (int)x.element_size(), idx_in ? idx_in->data_ptr() : nullptr, idx_mode,
+           idx_in ? (int)idx_in->element_size() : 4, static_cast<const int8_t*>(mask.data_ptr()),
+           out_x.numel() ? out_x.data_ptr() : nullptr, (out_idx && out_idx->numel()) ? out_idx->data_ptr() : nullptr,
+           n, out_x.numel(), ws.first, ws.second, stream_of(x)),
         op);
 }
 
@@ -693,7 +703,7 @@ at::Tensor run_compress(const at::Tensor& x_in, const at::Tensor& mask_in, int S
   at::Tensor x = prep(x_in), mask = mask_i8(mask_in, op);
   const int64_t k = count_ones(mask, op);
   at::Tensor z = at::empty({k}, x.options());
-  compress_into(x, nullptr, 0, mask, z, nullptr, op);
+  compress_into(x, nullptr, 0, mask, z, nullptr, op, true);
   return z;
 }
 
@@ -704,7 +714,7 @@ at::Tensor run_compress_pos(const at::Tensor& x_in, const at::Tensor& mask_in, i
   TORCH_CHECK(output_len >= 0, op, ": output_len must be non-negative");
   at::Tensor x = prep(x_in), mask = mask_i8(mask_in, op);
   at::Tensor z = at::empty({output_len}, x.options());
-  compress_into(x, nullptr, 0, mask, z, nullptr, op);
+  compress_into(x, nullptr, 0, mask, z, nullptr, op, false);
   return z;
 }
 
@@ -721,7 +731,7 @@ std::tuple<at::Tensor, at::Tensor> run_compress_ind(const at::Tensor& x_in, cons
   const int64_t k = count_ones(mask, op);
   at::Tensor z = at::empty({k}, x.options());
   at::Tensor zi = at::empty({k}, idx.options());
-  compress_into(x, &idx, 1, mask, z, &zi, op);
+  compress_into(x, &idx, 1, mask, z, &zi, op, true);
   return std::make_tuple(z, zi);
 }
 
@@ -734,7 +744,7 @@ std::tuple<at::Tensor, at::Tensor> run_compress_ind_no_arange(const at::Tensor& 
   const int64_t k = count_ones(mask, op);
   at::Tensor z = at::empty({k}, x.options());
   at::Tensor zi = at::empty({k}, x.options().dtype(at::kInt));
-  compress_into(x, nullptr, 2, mask, z, &zi, op);
+  compress_into(x, nullptr, 2, mask, z, &zi, op, true);
   return std::make_tuple(z, zi);
 }
 
@@ -757,9 +767,9 @@ at::Tensor run_seg_sum(const at::Tensor& x_in, const at::Tensor& f_in, int S) {
   if (k > 0) {
     at::Tensor ends_tail = ends.narrow(0, 1, k);
     // ends[1 + j] = (position of the j-th flag) + 1.
-    auto ws = workspace(x, tcs_compress_workspace_bytes(f.numel()));
+    auto ws = workspace(f, tcs_compress_workspace_bytes(f.numel()), true);
     const at::hip::HIPGuardMasqueradingAsCUDA guard(x.device());
-    check(tcs_compress(f.data_ptr(), 1, nullptr, 3, 4, static_cast<const int8_t*>(f.data_ptr()),
+    check(tcs_compress_counted(f.data_ptr(), 1, nullptr, 3, 4, static_cast<const int8_t*>(f.data_ptr()),
                        unused.data_ptr(), ends_tail.data_ptr(), f.numel(), k, ws.first, ws.second,
                        stream_of(x)),
           op);
@@ -769,6 +779,24 @@ at::Tensor run_seg_sum(const at::Tensor& x_in, const at::Tensor& f_in, int S) {
 }
 
 int abi_version() { return tcs_abi_version(); }
+
+// The batch tuner's cache: (kind, dtype code, log2n, S, tiles_per_wg,
+// method, window, ms) per entry, for tcuscan.tuning to persist.
+std::vector<std::tuple<int, int, int, int, int, int, int, double>> tuned_entries() {
+  std::vector<std::tuple<int, int, int, int, int, int, int, double>> out;
+  const int n = tcs_batch_tuned_count();
+  for (int i = 0; i < n; ++i) {
+    int k = 0, d = 0, l = 0, S = 0, t = 0, m = 0, w = 0;
+    float ms = -1.0f;
+    check(tcs_batch_tuned_entry(i, &k, &d, &l, &S, &t, &m, &w, &ms), "_tuned_entries");
+    out.emplace_back(k, d, l, S, t, m, w, (double)ms);
+  }
+  return out;
+}
+
+void set_tuned(int kind, int dtype, int log2n, int S, int tiles_per_wg, int method, int window, double ms) {
+  check(tcs_batch_tuned_set(kind, dtype, log2n, S, tiles_per_wg, method, window, (float)ms), "_set_tuned");
+}
 
 // Workspace of the current stream of like's device (created if needed), so
 // that tests can inspect and prepare its launch state.  The tensor shares
@@ -1249,6 +1277,10 @@ PYBIND11_MODULE(tcuscan_ops, m) {
   m.def("run_seg_sum_multi_cube", &tcuscan::run_seg_sum_multi_cube, py::arg("x"), py::arg("upper"),
         py::arg("lower_strict"), py::arg("indptr"), py::arg("segm_offsets") = py::none(),
         "Segmented Sum (multi-cube)");
+  m.def("_tuned_entries", &tcuscan::tuned_entries, "The batch tuner's cache of this process");
+  m.def("_set_tuned", &tcuscan::set_tuned, py::arg("kind"), py::arg("dtype"), py::arg("log2n"), py::arg("S"),
+        py::arg("tiles_per_wg"), py::arg("method"), py::arg("window"), py::arg("ms") = -1.0,
+        "Adds or replaces one entry of the batch tuner's cache");
   m.def("run_spmv_v2", &tcuscan::run_spmv_v2, py::arg("vals"), py::arg("indptr"), py::arg("cols"),
         py::arg("x"), py::arg("s"), py::arg("segm_offsets") = py::none(),
         "Sparse Matrix-Vector Multiplication Using Segmented Sum");

@@ -26,7 +26,8 @@ except ImportError as exc:  # pragma: no cover - exercised only without a build
 
 from tcuscan_ops import *  # noqa: E402,F401,F403
 
+from . import tuning  # noqa: E402
 from .sampling import sample_top_p, top_p_filter  # noqa: E402
 
 __all__ = [n for n in dir(tcuscan_ops) if n.startswith("run_")] + ["BatchPlan", "Server", "sample_top_p",
-                                                                   "top_p_filter"]
+                                                                   "top_p_filter", "tuning"]

@@ -143,3 +143,38 @@ def test_torch_graph_replays_of_plain_ops(ops, dev, dtype):
             gm.replay()
             torch.cuda.synchronize()
             assert_exact(to_np(zm), oracle.compress(to_np(xs), to_np(mask)), f"replay {rep}")
+
+
+def test_tuning_cache_persists(ops, dev, tmp_path):
+    """The tuner's cache goes to a JSON file and back: loaded entries apply
+    to plans built afterwards with no timing runs, and a file written for
+    another device is ignored."""
+    import json
+    import math
+
+    from tcuscan import tuning
+
+    sc = workloads.uneven_scans(40, 16, 18, seed=9, device=dev)
+    n = sc.x.numel()
+    log2n = math.ceil(math.log2(n))
+    ops._set_tuned(1, 0, log2n, 64, 2, 1, 32, 0.5)  # seg-scan, fp16: S=64, 2 tiles, DPP, window 32
+    path = tmp_path / "tune.json"
+    tuning.save(str(path))
+    d = json.loads(path.read_text())
+    assert d["device"]["compute_units"] == torch.cuda.get_device_properties(0).multi_processor_count
+    assert any(e["kind"] == 1 and e["dtype"] == 0 and e["log2n"] == log2n and e["S"] == 64 for e in d["entries"])
+    other = tmp_path / "other.json"
+    d2 = dict(d, device=dict(d["device"], name="another GPU"))
+    other.write_text(json.dumps(d2))
+    assert tuning.load(str(other)) == 0
+    ops._set_tuned(1, 0, log2n, 32, 16, 1, 8, 0.25)  # overwritten in this process ...
+    assert tuning.load(str(path)) == len(d["entries"])  # ... and restored from the file
+    plan = ops.BatchPlan()
+    y = plan.add_seg_scan(sc.x, sc.flags)
+    plan.tune(reps=1)  # a cached key: no timing runs
+    S, tpw, method, window, ms = plan.configs()[0]
+    assert (S, tpw, method, window) == (64, 2, "dpp", 32) and abs(ms - 0.5) < 1e-6
+    plan.instantiate(max_branches=1)
+    plan.launch()
+    torch.cuda.synchronize()
+    assert_exact(to_np(y), oracle.seg_scan_flags(to_np(sc.x), to_np(sc.flags)), "plan with a loaded config")

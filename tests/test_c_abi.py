@@ -8,15 +8,20 @@ import pytest
 
 REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(REPO, "include", "tcuscan.h")
+HEADERS = sorted(os.path.join(REPO, "include", f) for f in os.listdir(os.path.join(REPO, "include"))
+                 if f.endswith(".h"))
 LIB = os.path.join(REPO, "pytorch-tcuscan_amd", "libtcuscan_hip.so")
 
 TCS_OK, TCS_ERR_INVALID_ARG, TCS_ERR_UNSUPPORTED_DTYPE, TCS_ERR_WORKSPACE = 0, 1, 2, 3
 
 
 def declared_functions():
-    text = open(HEADER).read()
-    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(tcs_[a-z0-9_]+)\s*\(", text)))
+    """Every function declared in include/*.h."""
+    names = set()
+    for h in HEADERS:
+        text = re.sub(r"/\*.*?\*/", "", open(h).read(), flags=re.S)
+        names.update(re.findall(r"\b(tcs_[a-z0-9_]+)\s*\(", text))
+    return sorted(names)
 
 
 @pytest.fixture(scope="module")
@@ -124,3 +129,23 @@ def test_workspace_maintenance_validation(lib):
     assert reset(fake, 64, None) == TCS_ERR_WORKSPACE          # smaller than the header
     assert polls(None, 1 << 20, 0, None) == TCS_ERR_WORKSPACE
     assert polls(fake, 1 << 20, 513, None) == TCS_ERR_INVALID_ARG  # above the default limit
+
+
+def test_tuner_cache_entries_are_validated(lib):
+    f = lib.tcs_batch_tuned_set
+    f.argtypes = [ctypes.c_int] * 7 + [ctypes.c_float]
+    assert f(1, 0, 20, 32, 16, 1, 8, 1.0) == TCS_OK
+    assert f(0, 3, 28, 128, 4, 2, 32, 1.0) == TCS_OK          # int8 scan on the matrix units
+    assert f(1, 0, 20, 48, 16, 1, 8, 1.0) == TCS_ERR_INVALID_ARG  # S not 32/64/128
+    assert f(1, 0, 20, 32, 3, 1, 8, 1.0) == TCS_ERR_INVALID_ARG   # tiles_per_wg not a power of two
+    assert f(1, 0, 20, 32, 16, 3, 8, 1.0) == TCS_ERR_INVALID_ARG  # no such method
+    assert f(1, 0, 20, 32, 16, 1, 65, 1.0) == TCS_ERR_INVALID_ARG  # window past a wave
+    assert f(2, 0, 20, 32, 16, 1, 8, 1.0) == TCS_ERR_INVALID_ARG   # seg-sums are not tuned
+    assert f(1, 0, 20, 32, 16, 2, 8, 1.0) == TCS_ERR_UNSUPPORTED_DTYPE  # no matrix-unit seg-scan
+    n = lib.tcs_batch_tuned_count()
+    got = []
+    for i in range(n):
+        v = [ctypes.c_int() for _ in range(7)] + [ctypes.c_float()]
+        assert lib.tcs_batch_tuned_entry(i, *[ctypes.byref(x) for x in v]) == TCS_OK
+        got.append(tuple(x.value for x in v[:7]))
+    assert (1, 0, 20, 32, 16, 1, 8) in got and (0, 3, 28, 128, 4, 2, 32) in got

@@ -100,6 +100,9 @@ def test_compaction_fallback(ops, dev, fallback, p):
     ez2, ei2 = oracle.compress_ind_no_arange(to_np(x), to_np(m))
     assert_exact(to_np(z2), ez2, "no_arange values")
     assert_exact(to_np(zi2), ei2, "no_arange indices")
+    # run_compress_pos has no count pass: its offsets come by look-back.
+    z3 = ops.run_compress_pos(x.to(dev), m.to(dev), ez.size, 128)
+    assert_exact(to_np(z3), ez, f"compress_pos values p={p}")
 
 
 @pytest.mark.parametrize("pivot", [-3.0, 0.0, 0.5, 3.0])
@@ -150,6 +153,26 @@ def test_spmv_fallback(ops, dev, fallback):
     col = col[np.lexsort((col, row_id))].astype(np.int32)  # sorted within each row
     vals = uniform_pm1(nnz, torch.float16, 11)
     xv = uniform_pm1(cols, torch.float16, 12)
+    y = ops.run_spmv_v2(vals.to(dev), torch.from_numpy(indptr).to(dev), torch.from_numpy(col).to(dev),
+                        xv.to(dev), 128, None)
+    exp = oracle.spmv(to_np(vals), indptr, col, to_np(xv))
+    np.testing.assert_allclose(to_np(y), exp, rtol=1e-6, atol=1e-6)
+
+
+def test_spmv_split_fallback(ops, dev, fallback):
+    """The split gather (a 2^22-column fp16 vector): both parts' chains
+    through the fallback, which sums only its own part's products."""
+    rng = np.random.default_rng(6)
+    rows, cols = 200_003, 1 << 22
+    lens = oracle.lognormal_row_lengths(rows, 16.0, 1.2, 20000, rng)
+    lens[11] = 20000
+    indptr = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    nnz = int(indptr[-1])
+    row_id = np.repeat(np.arange(rows), lens)
+    col = rng.integers(0, cols, nnz)
+    col = col[np.lexsort((col, row_id))].astype(np.int32)
+    vals = uniform_pm1(nnz, torch.float16, 13)
+    xv = uniform_pm1(cols, torch.float16, 14)
     y = ops.run_spmv_v2(vals.to(dev), torch.from_numpy(indptr).to(dev), torch.from_numpy(col).to(dev),
                         xv.to(dev), 128, None)
     exp = oracle.spmv(to_np(vals), indptr, col, to_np(xv))
