@@ -451,6 +451,37 @@ def _spmv_workload(rows: int, steps: int, warmup: int):
     return out
 
 
+def _segsum_workload(nnz_log2: int, steps: int, warmup: int, dtype_name: str = "int8"):
+    """run_seg_sum_multi_core over 2^nnz_log2 values (int8 in [-4, 4] or fp16
+    U[-1, 1]) in segments of geometric length (mean 64), full CSR indptr.
+    Bytes: the values, plus 4 per row end read and 4 per row written."""
+    import numpy as np
+    import torch
+
+    import tcuscan
+
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(0)
+    n = 1 << nnz_log2
+    lens = rng.geometric(1 / 64, size=int(n / 64 * 1.1))
+    ends = np.cumsum(lens)
+    ends = ends[ends < n]
+    indptr = torch.from_numpy(np.concatenate([[0], ends, [n]]).astype(np.int32)).to(dev)
+    g = torch.Generator(device=dev).manual_seed(0)
+    if dtype_name == "int8":
+        x = torch.randint(-4, 5, (n,), generator=g, device=dev, dtype=torch.int8)
+    else:
+        x = (torch.rand(n, generator=g, device=dev) * 2 - 1).half()
+    rows = indptr.numel() - 1
+    torch.cuda.synchronize()
+    m = _measure(lambda: tcuscan.run_seg_sum_multi_core(x, indptr, 128), steps, warmup, flush=False)
+    bytes_per_step = n * x.element_size() + 8 * rows + 4
+    del x, indptr
+    torch.cuda.empty_cache()
+    return {"nnz": n, "rows": rows, "bytes_per_step": bytes_per_step, "wall_s": m["wall_s"],
+            "hot": m["hot"], "gbs": bytes_per_step / (m["hot"]["mean_ms"] * 1e-3) / 1e9}
+
+
 def _copy_workload(n: int, steps: int, warmup: int):
     """A plain float32 copy of n elements (PMC calibration: 4n bytes read
     and 4n written per launch)."""
@@ -488,8 +519,11 @@ def _batched_workload(steps: int, warmup: int, num_scans: int = 10_000, num_segs
     y_scan = plan.add_seg_scan(sc.x, sc.flags)
     y_sum = plan.add_seg_sum(ss.x, ss.indptr)
     t0 = time.perf_counter()
+    loaded = None
     if tune_cache:
-        tcuscan.tuning.tune(plan, reps=2, path=tune_cache)
+        loaded = tcuscan.tuning.load(tune_cache)
+        plan.tune(reps=2)
+        tcuscan.tuning.save(tune_cache)
     else:
         plan.tune(reps=2)
     tune_s = time.perf_counter() - t0
@@ -508,7 +542,8 @@ def _batched_workload(steps: int, warmup: int, num_scans: int = 10_000, num_segs
     if not per_call_leg:
         return {"scans": num_scans, "scan_elems": total, "segsums": num_segsums, "segsum_nnz": nnz,
                 "bytes_per_step": bytes_per_step, "batched_ms": ms_b, "batched_event": mb["hot"],
-                "gbs_batched": bytes_per_step / (ms_b * 1e-3) / 1e9, "tuned": tuned, "tune_s": tune_s}
+                "gbs_batched": bytes_per_step / (ms_b * 1e-3) / 1e9, "tuned": tuned, "tune_s": tune_s,
+                "tune_cache_entries_loaded": loaded, "device_key": tcuscan.tuning.device_key()}
 
     scan_pieces = sc.pieces()
     sum_pieces = ss.pieces()
@@ -765,7 +800,7 @@ def main() -> int:
     ap.add_argument("--only", choices=["scan_f16", "scan_i8", "scan_bf16", "scan_f16_mfma", "scan_i8_mfma",
                                        "scan_f16_dpp", "scan_i8_dpp", "compress_ind", "spmv", "copy_f32",
                                        "batched", "serve", "filter_ge", "searchsorted", "sort_f16", "sort_i32",
-                                       "topk_f16", "top_p", "stream_ceilings"],
+                                       "topk_f16", "top_p", "stream_ceilings", "segsum_i8", "segsum_f16"],
                     help="run one workload and print its timing only (for profilers)")
     a = ap.parse_args()
     if a.gpus != 1:
@@ -784,6 +819,8 @@ def main() -> int:
             r = _compress_workload(n, a.S, a.steps, a.warmup)
         elif a.only == "spmv":
             r = _spmv_workload(1 << 22, a.steps, a.warmup)
+        elif a.only in ("segsum_i8", "segsum_f16"):
+            r = _segsum_workload(26, a.steps, a.warmup, "int8" if a.only == "segsum_i8" else "fp16")
         elif a.only == "stream_ceilings":
             r = _stream_ceilings()
         elif a.only == "batched":
@@ -873,6 +910,13 @@ def main() -> int:
         extras["searchsorted_i32_2p24_into_2p24"] = {
             "value": round(ss["gneedles_per_s"], 3), "unit": "G needles/s",
             "ms_per_step": round(ss["wall_s"] * 1e3 / ksteps, 4),
+        }
+        _progress("seg-sum")
+        sg = _segsum_workload(26, ksteps, max(3, a.warmup // 2))
+        extras["seg_sum_i8_2p26_geometric64"] = {
+            "value": round(sg["gbs"], 1), "unit": "GB/s", "nnz": sg["nnz"], "rows": sg["rows"],
+            "ms_per_call": round(sg["hot"]["mean_ms"], 4), "p99_ms": round(sg["hot"]["p99_ms"], 4),
+            "note": "run_seg_sum_multi_core, int8 values, geometric segments (mean 64)",
         }
         _progress("spmv")
         sp = _spmv_workload(1 << 22, ksteps, max(3, a.warmup // 2))

@@ -49,7 +49,13 @@ struct ScanGeom {
 template <class C, int J>
 struct TileRegs {
   Packed<typename C::In, C::G> raw[J];
-  uint32_t heads[J];
+  // Segment heads: bit i of group j is bit (G * j + i) % 32 of word
+  // (G * j) / 32 (groups of G <= 16 items; packed so that the heads of a
+  // whole tile take few registers).
+  uint32_t heads[(J * C::G + 31) / 32];
+  __device__ __forceinline__ uint32_t head_bits(int j) const {
+    return (heads[(C::G * j) >> 5] >> ((C::G * j) & 31)) & ((1u << C::G) - 1u);
+  }
 };
 
 template <class C, int SEG, int G>
@@ -169,8 +175,11 @@ __device__ __forceinline__ void tile_load(TileRegs<C, J>& r, const typename C::I
     load_items<typename C::In, Gm::G, FULL>(x, wave0 + group_offset<C, LOC>(j, lane), a.n, r.raw[j]);
   const int64_t wbase = wave0 + (int64_t)lane * Gm::G;
 #pragma unroll
+  for (int w = 0; w < (J * Gm::G + 31) / 32; ++w) r.heads[w] = 0;
+#pragma unroll
   for (int j = 0; j < J; ++j)
-    r.heads[j] = group_heads<C, SEG, Gm::G>(flags, wbase + j * Gm::GSPAN, a.n, a.seg_len, FULL);
+    r.heads[(Gm::G * j) >> 5] |=
+        group_heads<C, SEG, Gm::G>(flags, wbase + j * Gm::GSPAN, a.n, a.seg_len, FULL) << ((Gm::G * j) & 31);
 }
 
 // Data-wave reduction of a tile: lane-serial sums, one wave scan per group,
@@ -179,13 +188,14 @@ __device__ __forceinline__ void tile_load(TileRegs<C, J>& r, const typename C::I
 // for tiles holding a segment head, aggregate otherwise.
 template <class C, int NW, int J, int SEG, int LOC>
 __device__ __forceinline__ void tile_reduce(const TileRegs<C, J>& r, typename C::Acc (&gex)[J],
-                                            int (&gexf)[J], PipeSlot<C, NW>& sl,
+                                            uint32_t& gexf, PipeSlot<C, NW>& sl,
                                             typename C::Acc start, void* ws, const ScanArgs& a,
                                             uint32_t tile, int wid, int lane) {
   using Acc = typename C::Acc;
   constexpr int G = C::G;
   constexpr bool kSeg = SEG != kSegNone;
   static_assert(LOC == kLocDpp || !kSeg, "the matrix-unit scan covers plain scans only");
+  gexf = 0u;
   Acc wtot = Acc(0);
   int wtotf = 0;
   if constexpr (LOC == kLocMfma) {
@@ -200,31 +210,39 @@ __device__ __forceinline__ void tile_reduce(const TileRegs<C, J>& r, typename C:
   }
 #pragma unroll
   for (int j = 0; j < (LOC == kLocMfma ? 0 : J); ++j) {
+    // Segment heads are usually rare: a group row of the wave with none
+    // takes the plain scan (no per-item selects, no flag steps), which
+    // performs the same additions in the same order.
+    const bool seg_here = kSeg && __ballot(r.head_bits(j) != 0u) != 0ull;
     Acc tv = Acc(0);
-    int tf = 0;
+    int wf = 0;
+    Acc wv;
+    if (!seg_here) {
 #pragma unroll
-    for (int i = 0; i < G; ++i) {
-      const Acc v = C::conv(r.raw[j][i]);
-      if (kSeg && (r.heads[j] >> i & 1u)) {
-        tv = v;
-        tf = 1;
-      } else {
-        tv = tv + v;
-      }
-    }
-    int wf = tf;
-    Acc wv = tv;
-    if constexpr (kSeg) {
-      wave_seg_incl_scan(wf, wv);
+      for (int i = 0; i < G; ++i) tv = tv + C::conv(r.raw[j][i]);
+      wv = wave_incl_scan(tv);
     } else {
-      wv = wave_incl_scan(wv);
+      int tf = 0;
+#pragma unroll
+      for (int i = 0; i < G; ++i) {
+        const Acc v = C::conv(r.raw[j][i]);
+        if (r.head_bits(j) >> i & 1u) {
+          tv = v;
+          tf = 1;
+        } else {
+          tv = tv + v;
+        }
+      }
+      wf = tf;
+      wv = tv;
+      wave_seg_incl_scan(wf, wv);
     }
     const Acc ev = dpp_mov<kWaveShr1>(wv);
-    const int ef = kSeg ? dpp_i32<kWaveShr1>(wf) : 0;
+    const int ef = seg_here ? dpp_i32<kWaveShr1>(wf) : 0;
     gex[j] = ef ? ev : wtot + ev;
-    gexf[j] = wtotf | ef;
+    gexf |= (uint32_t)((wtotf | ef) != 0) << j;
     const Acc gt = readlane(wv, kWave - 1);
-    const int gf = kSeg ? readlane(wf, kWave - 1) : 0;
+    const int gf = seg_here ? readlane(wf, kWave - 1) : 0;
     wtot = gf ? gt : wtot + gt;
     wtotf |= gf;
   }
@@ -310,7 +328,7 @@ struct OutStage {
 // Second pass over the tile: add the carries and store.
 template <class C, int NW, int J, int SEG, bool FULL>
 __device__ __forceinline__ void tile_output(TileRegs<C, J>& r, const typename C::Acc (&gex)[J],
-                                            const int (&gexf)[J], const PipeSlot<C, NW>& sl,
+                                            const uint32_t gexf, const PipeSlot<C, NW>& sl,
                                             typename C::Out* __restrict__ y, int64_t n,
                                             uint32_t tile, int wid, int lane,
                                             uint32_t* __restrict__ os) {
@@ -331,17 +349,25 @@ __device__ __forceinline__ void tile_output(TileRegs<C, J>& r, const typename C:
   const int64_t wbase = (int64_t)tile * Gm::TILE + (int64_t)wid * Gm::WSEG + (int64_t)lane * G;
 #pragma unroll
   for (int j = 0; j < J; ++j) {
-    Acc run = gexf[j] ? gex[j] : cin + gex[j];
+    Acc run = (gexf >> j & 1u) ? gex[j] : cin + gex[j];
     Out o[G];
+    if (!(kSeg && __ballot(r.head_bits(j) != 0u) != 0ull)) {  // as in tile_reduce
 #pragma unroll
-    for (int i = 0; i < G; ++i) {
-      const Acc v = C::conv(r.raw[j][i]);
-      if (kSeg && (r.heads[j] >> i & 1u)) {
-        run = v;
-      } else {
-        run = run + v;
+      for (int i = 0; i < G; ++i) {
+        run = run + C::conv(r.raw[j][i]);
+        o[i] = C::out(run);
       }
-      o[i] = C::out(run);
+    } else {
+#pragma unroll
+      for (int i = 0; i < G; ++i) {
+        const Acc v = C::conv(r.raw[j][i]);
+        if (r.head_bits(j) >> i & 1u) {
+          run = v;
+        } else {
+          run = run + v;
+        }
+        o[i] = C::out(run);
+      }
     }
     if constexpr (FULL && OutStage<C, kLocDpp>::kOn) {
       u32x4 v0, v1;
@@ -455,7 +481,7 @@ __device__ __forceinline__ void load_any(TileRegs<C, J>& r, const typename C::In
 
 template <class C, int NW, int J, int SEG, int LOC>
 __device__ __forceinline__ void output_any(TileRegs<C, J>& r, const typename C::Acc (&gex)[J],
-                                           const int (&gexf)[J], const PipeSlot<C, NW>& sl,
+                                           const uint32_t gexf, const PipeSlot<C, NW>& sl,
                                            typename C::Out* __restrict__ y, const ScanArgs& a,
                                            uint32_t tile, int wid, int lane,
                                            uint16_t* __restrict__ stage, uint32_t* __restrict__ os) {
@@ -533,7 +559,7 @@ __global__ __launch_bounds__((NW + 1) * kWave) void scan_kernel(
 
   TileRegs<C, J> reg[2];
   Acc gex[2][J];
-  int gexf[2][J];
+  uint32_t gexf[2];  // bit j: group j's exclusive value starts a segment
   uint16_t* stage = stage_all[lb_wave ? 0 : wid];
   uint32_t* ostage = ostage_all[lb_wave ? 0 : wid];
   if (!lb_wave) {

@@ -40,8 +40,11 @@
 
 namespace tcs {
 
-constexpr int kSegNW = 8, kSegJ = 2, kSegG = 8;
-constexpr int64_t kSegTile = (int64_t)kSegNW * kSegJ * kWave * kSegG;  // 8192
+// Tiles of kSegNW waves x J groups x 64 lanes x G items; each value source
+// picks J and G (16 bytes per group load).  kSegMinTile, the smallest tile
+// any source uses, sizes the workspaces.
+constexpr int kSegNW = 8;
+constexpr int64_t kSegMinTile = 8192;
 
 template <class IdxT>
 __device__ __forceinline__ int64_t row_end(const IdxT* __restrict__ ends, int64_t r, int64_t n_ends,
@@ -55,7 +58,7 @@ __device__ __forceinline__ int64_t row_end(const IdxT* __restrict__ ends, int64_
 // marks a tile past the last row.
 // With `part` set (the split SpMV) empty rows zero both partial sums
 // part[r] and part[R + r] instead of y[r].
-template <class IdxT, class OutT>
+template <int64_t TILE, class IdxT, class OutT>
 __global__ __launch_bounds__(256) void seg_rows_kernel(const IdxT* __restrict__ ends, int64_t R,
                                                        int64_t n_ends, int64_t nnz,
                                                        OutT* __restrict__ y, double* __restrict__ part,
@@ -73,25 +76,31 @@ __global__ __launch_bounds__(256) void seg_rows_kernel(const IdxT* __restrict__ 
       y[r] = OutT(0);
     }
   } else {
-    for (int64_t b = (s + kSegTile - 1) / kSegTile; b * kSegTile < e && b < num_tiles; ++b)
+    for (int64_t b = (s + TILE - 1) / TILE; b * TILE < e && b < num_tiles; ++b)
       tile_row[b] = (uint64_t)r;
   }
   if (r == R - 1) {
     const int64_t last = e > s ? e : s;
-    for (int64_t b = (last + kSegTile - 1) / kSegTile; b < num_tiles; ++b) tile_row[b] = (uint64_t)R;
+    for (int64_t b = (last + TILE - 1) / TILE; b < num_tiles; ++b) tile_row[b] = (uint64_t)R;
   }
 }
 
-// Value sources.
+// Value sources.  Plain values: 16-byte groups (16 int8 or 8 fp16 / bf16
+// values; fp32 takes two loads per group of 8); 16K-element tiles for int8
+// (one tile's chain of dependent loads then covers 16 KiB instead of 8),
+// 8K for wider values.
 template <class C>
 struct SrcPlain {
   using Acc = typename C::Acc;
   using In = typename C::In;
+  static constexpr int G = sizeof(In) == 1 ? 16 : 8;
+  static constexpr int J = 2;
+  static constexpr int64_t kTile = (int64_t)kSegNW * J * kWave * G;
   const In* __restrict__ x;
-  struct Regs { In v[kSegJ][kSegG]; };
+  struct Regs { Packed<In, G> v[J]; };
   template <bool FULL>
   __device__ __forceinline__ void load(Regs& r, int j, int64_t gb, int64_t t1) const {
-    load_items<In, kSegG, FULL>(x, gb, t1, r.v[j]);
+    load_items<In, G, FULL>(x, gb, t1, r.v[j]);
   }
   template <bool FULL>
   __device__ __forceinline__ void gather(Regs&, int, int64_t, int64_t) const {}
@@ -99,7 +108,7 @@ struct SrcPlain {
   __device__ __forceinline__ Acc value_at(int64_t p) const { return C::conv(x[p]); }
   __device__ __forceinline__ void barrier(Regs& r) const {
 #pragma unroll
-    for (int j = 0; j < kSegJ; ++j) reg_barrier(r.v[j]);
+    for (int j = 0; j < J; ++j) reg_barrier(r.v[j]);
   }
   __device__ __forceinline__ void set_part(int, int) {}
   static constexpr bool kSplittable = false;
@@ -120,11 +129,15 @@ struct SrcSpmv {
   int64_t x_len;
   uint32_t c_lo = 0, c_n = 0;
   static constexpr bool kSplittable = true;
+  // 8K-element tiles (measured: 4K tiles 0.639 vs 0.563 ms on the 2^22
+  // lognormal matrix).
+  static constexpr int G = 8, J = 2;
+  static constexpr int64_t kTile = (int64_t)kSegNW * J * kWave * G;
   struct Regs {
-    Packed<In, kSegG> v[kSegJ];
-    Packed<In, kSegG> g[kSegJ];
-    uint32_t c[kSegJ][kSegG];  // column indices, from load() until gather()
-    uint32_t ok[kSegJ];        // bit i: item i of group j contributes
+    Packed<In, G> v[J];
+    Packed<In, G> g[J];
+    uint32_t c[J][G];  // column indices, from load() until gather()
+    uint32_t ok[J];        // bit i: item i of group j contributes
   };
   // Part `part` of `parts` equal column ranges (the last one shorter).
   __device__ __forceinline__ void set_part(int part, int parts) {
@@ -142,7 +155,7 @@ struct SrcSpmv {
   // the caches.
   template <class T, int W>
   __device__ __forceinline__ static void load_nt(const T* __restrict__ p, int64_t base, uint32_t (&w)[W]) {
-    static_assert(W * 4 == kSegG * (int)sizeof(T) && W % 4 == 0, "whole 16-byte vectors");
+    static_assert(W * 4 == G * (int)sizeof(T) && W % 4 == 0, "whole 16-byte vectors");
     const u32x4* vp = reinterpret_cast<const u32x4*>(p + base);
 #pragma unroll
     for (int k = 0; k < W / 4; ++k) {
@@ -162,18 +175,18 @@ struct SrcSpmv {
       load_nt<In>(vals, gb, r.v[j].w);
       load_nt<int32_t>(cols, gb, r.c[j]);
     } else {
-      load_items<In, kSegG, false>(vals, gb, t1, r.v[j]);
-      int32_t c[kSegG];
-      load_items<int32_t, kSegG, false>(cols, gb, t1, c);
+      load_items<In, G, false>(vals, gb, t1, r.v[j]);
+      int32_t c[G];
+      load_items<int32_t, G, false>(cols, gb, t1, c);
       __builtin_memcpy(r.c[j], c, sizeof(c));
     }
   }
   template <bool FULL>
   __device__ __forceinline__ void gather(Regs& r, int j, int64_t gb, int64_t t1) const {
-    In g[kSegG];
+    In g[G];
     uint32_t ok = 0;
 #pragma unroll
-    for (int i = 0; i < kSegG; ++i) {
+    for (int i = 0; i < G; ++i) {
       const bool k = in_range((int32_t)r.c[j][i]) && (FULL || gb + i < t1);
       // A 32-bit byte offset from a uniform base (x_len < 2^31): one VGPR
       // per address instead of a 64-bit pointer.
@@ -201,7 +214,7 @@ struct SrcSpmv {
   }
   __device__ __forceinline__ void barrier(Regs& r) const {
 #pragma unroll
-    for (int j = 0; j < kSegJ; ++j) {
+    for (int j = 0; j < J; ++j) {
       reg_barrier(r.v[j]);
       reg_barrier(r.g[j]);
     }
@@ -218,11 +231,19 @@ __global__ __launch_bounds__(kSegNW * kWave) void seg_reduce_kernel(
     OutT* __restrict__ y, double* __restrict__ part_out, const uint64_t* __restrict__ tile_row,
     uint32_t num_tiles, void* ws) {
   using Acc = typename Src::Acc;
-  constexpr int G = kSegG, J = kSegJ, NW = kSegNW;
+  constexpr int G = Src::G, J = Src::J, NW = kSegNW;
+  constexpr int64_t TILE = Src::kTile;
   constexpr int64_t GSPAN = (int64_t)kWave * G, WSEG = GSPAN * J;
-  __shared__ uint32_t s_head[kSegTile / 32];  // bit q: a row starts at t0 + q
-  __shared__ uint32_t s_endb[kSegTile / 32];  // bit q: a row ends at t0 + q (inclusive)
-  __shared__ uint32_t s_rowk[kSegTile];       // at an end position: r - (r_lo - 1) of that row
+  static_assert(32 % G == 0 || G == 16, "a group's bits lie in one 32-bit word");
+  __shared__ uint32_t s_head[TILE / 32];  // bit q: a row starts at t0 + q
+  __shared__ uint32_t s_endb[TILE / 32];  // bit q: a row ends at t0 + q (inclusive)
+  // At an end position q: k = r - (r_lo - 1) of the row ending there.
+  // Tiles above 8K keep k in 16 bits (half the LDS) when the tile's rows
+  // number at most 65536 (k16); otherwise each end's row is found by a
+  // binary search over the row ends.
+  constexpr bool kRowK16 = TILE > 8192;
+  using RowK = typename std::conditional<kRowK16, uint16_t, uint32_t>::type;
+  __shared__ RowK s_rowk[TILE];
   __shared__ Acc s_wval[NW], s_wexcl[NW], s_carry;
   __shared__ int s_wflag[NW], s_wexclf[NW];
   __shared__ LaunchTag s_tag;
@@ -234,17 +255,17 @@ __global__ __launch_bounds__(kSegNW * kWave) void seg_reduce_kernel(
   const uint32_t tile = blockIdx.x / SPLIT;
   const int part = SPLIT > 1 ? (int)(blockIdx.x % SPLIT) : 0;
   src.set_part(part, SPLIT);
-  const int64_t t0 = (int64_t)tile * kSegTile;
+  const int64_t t0 = (int64_t)tile * TILE;
   // Loads that do not depend on one another go out together: the last row
   // end, the tile's first rows, and the tile's values (bounded by nnz;
   // values past the last row end belong to no row and are never stored).
   const int64_t end_all = row_end(ends, R - 1, n_ends, nnz);
   int64_t r_lo = (int64_t)tile_row[tile];
   int64_t r_hi = tile + 1 < num_tiles ? (int64_t)tile_row[tile + 1] : R - 1;
-  const int64_t t1 = t0 + kSegTile < nnz ? t0 + kSegTile : nnz;
+  const int64_t t1 = t0 + TILE < nnz ? t0 + TILE : nnz;
   typename Src::Regs regs;
   const int64_t wbase = t0 + (int64_t)wid * WSEG + (int64_t)lane * G;
-  const bool full = t0 + kSegTile <= nnz;
+  const bool full = t0 + TILE <= nnz;
   if (full) {
 #pragma unroll
     for (int j = 0; j < J; ++j) src.template load<true>(regs, j, wbase + j * GSPAN, t1);
@@ -261,6 +282,7 @@ __global__ __launch_bounds__(kSegNW * kWave) void seg_reduce_kernel(
   if (r_hi > R - 1) r_hi = R - 1;
   if (r_hi < r_lo) r_hi = r_lo;
   const int64_t cnt = r_hi - r_lo + 2;
+  const bool k16 = !kRowK16 || cnt <= 65536;
   constexpr int kEarly = 2;
   int64_t e_early[kEarly], ep_early[kEarly];
 #pragma unroll
@@ -280,7 +302,7 @@ __global__ __launch_bounds__(kSegNW * kWave) void seg_reduce_kernel(
   // Mark the boundaries.  Among equal ends (empty rows) only the first, the
   // row that holds the element before it, marks them, so every position is
   // written by at most one thread.
-  for (int k = threadIdx.x; k < kSegTile / 32; k += NW * kWave) {
+  for (int k = threadIdx.x; k < TILE / 32; k += NW * kWave) {
     s_head[k] = 0;
     s_endb[k] = 0;
   }
@@ -294,7 +316,7 @@ __global__ __launch_bounds__(kSegNW * kWave) void seg_reduce_kernel(
     if (e > t0 && e <= t1) {
       const int q = (int)(e - 1 - t0);
       atomicOr(&s_endb[q >> 5], 1u << (q & 31));
-      s_rowk[q] = (uint32_t)k;
+      if (k16) s_rowk[q] = (RowK)k;
     }
   };
 #pragma unroll
@@ -311,8 +333,9 @@ __global__ __launch_bounds__(kSegNW * kWave) void seg_reduce_kernel(
 #pragma unroll
   for (int j = 0; j < J; ++j) {
     const int q = (int)(wbase + j * GSPAN - t0);
-    hmask[j] = (s_head[q >> 5] >> (q & 31)) & 0xffu;
-    emask[j] = (s_endb[q >> 5] >> (q & 31)) & 0xffu;
+    constexpr uint32_t kGroupBits = G == 32 ? ~0u : ((1u << G) - 1u);
+    hmask[j] = (s_head[q >> 5] >> (q & 31)) & kGroupBits;
+    emask[j] = (s_endb[q >> 5] >> (q & 31)) & kGroupBits;
   }
 
   // Segmented reduction of the tile (as in the scans).
@@ -367,8 +390,8 @@ __global__ __launch_bounds__(kSegNW * kWave) void seg_reduce_kernel(
     const bool first_head = readlane((int)(hmask[0] & 1u), 0) != 0;
     // Fallback: {row head inside tile u, sum since its last head}.
     auto fb = [&](int64_t u) {
-      const int64_t u0 = u * kSegTile;
-      const int64_t u1 = u0 + kSegTile < end_all ? u0 + kSegTile : end_all;
+      const int64_t u0 = u * TILE;
+      const int64_t u1 = u0 + TILE < end_all ? u0 + TILE : end_all;
       // Row holding u1 - 1: first r with end(r) > u1 - 1.
       int64_t lo = 0, hi = R - 1;
       while (lo < hi) {
@@ -419,7 +442,18 @@ __global__ __launch_bounds__(kSegNW * kWave) void seg_reduce_kernel(
         run = run + v;
       }
       if (emask[j] >> i & 1u) {
-        int64_t row = r_lo - 1 + (int64_t)s_rowk[q0 + i];
+        int64_t row;
+        if (!kRowK16 || k16) {
+          row = r_lo - 1 + (int64_t)s_rowk[q0 + i];
+        } else {
+          // The row holding position t0 + q0 + i: the first whose end is past it.
+          int64_t lo = r_lo - 1 < 0 ? 0 : r_lo - 1, hi = r_hi;
+          while (lo < hi) {
+            const int64_t mid = (lo + hi) >> 1;
+            if (row_end(ends, mid, n_ends, nnz) <= t0 + q0 + i) lo = mid + 1; else hi = mid;
+          }
+          row = lo;
+        }
         if (row < 0) row = 0;
         if (row > R - 1) row = R - 1;
         if constexpr (SPLIT > 1) {
@@ -456,9 +490,11 @@ struct SegSumPlan {
   size_t part_off;
 };
 
-inline SegSumPlan seg_plan(int64_t nnz, int parts = 1, int64_t rows = 0) {
+// tile: the value source's tile (kSegMinTile, the smallest, sizes the
+// workspaces a caller allocates without knowing the source).
+inline SegSumPlan seg_plan(int64_t nnz, int parts = 1, int64_t rows = 0, int64_t tile = kSegMinTile) {
   SegSumPlan p;
-  p.num_tiles = (uint32_t)((nnz + kSegTile - 1) / kSegTile);
+  p.num_tiles = (uint32_t)((nnz + tile - 1) / tile);
   p.parts = parts;
   const size_t lb = lookback_ws_bytes((uint64_t)p.num_tiles * (uint64_t)parts);
   p.tile_row_off = lb;
@@ -479,16 +515,17 @@ inline int spmv_parts(int64_t x_len, int vals_dtype) {
 template <class Src, class IdxT, class OutT>
 static int launch_seg(const Src& src, const void* ends_v, int64_t R, int64_t n_ends, int64_t nnz,
                       void* y_v, void* ws, size_t ws_bytes, hipStream_t st, int parts = 1) {
-  SegSumPlan plan = seg_plan(nnz, parts, R);
+  static_assert(Src::kTile % kSegMinTile == 0, "tiles are multiples of the smallest tile");
+  SegSumPlan plan = seg_plan(nnz, parts, R, Src::kTile);
   // One chain when there is nothing to gather or no room for the split.
-  if (parts > 1 && (nnz == 0 || !ws || ws_bytes < plan.ws_bytes)) plan = seg_plan(nnz);
+  if (parts > 1 && (nnz == 0 || !ws || ws_bytes < plan.ws_bytes)) plan = seg_plan(nnz, 1, 0, Src::kTile);
   if (!ws || ws_bytes < plan.ws_bytes) return TCS_ERR_WORKSPACE;
   const auto* ends = static_cast<const IdxT*>(ends_v);
   auto* y = static_cast<OutT*>(y_v);
   auto* tile_row = reinterpret_cast<uint64_t*>(static_cast<char*>(ws) + plan.tile_row_off);
   double* part = plan.parts > 1 ? reinterpret_cast<double*>(static_cast<char*>(ws) + plan.part_off) : nullptr;
   const int64_t row_blocks = (R + 255) / 256;
-  hipLaunchKernelGGL((seg_rows_kernel<IdxT, OutT>), dim3((uint32_t)row_blocks), dim3(256), 0, st,
+  hipLaunchKernelGGL((seg_rows_kernel<Src::kTile, IdxT, OutT>), dim3((uint32_t)row_blocks), dim3(256), 0, st,
                      ends, R, n_ends, nnz, y, part, tile_row, plan.num_tiles);
   TCS_CHECK_LAUNCH();
   if (plan.num_tiles == 0) return TCS_OK;

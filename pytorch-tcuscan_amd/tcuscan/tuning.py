@@ -4,7 +4,7 @@ BatchPlan.tune() times candidate tilings of every scan op on the device and
 keeps the fastest per (op kind, dtype, ceil(log2 n)) for the process.  These
 helpers write that cache to a JSON file and read it back in a later
 process, so a plan built there skips the timing runs.  The file records the
-device it was measured on (name and compute-unit count); entries from
+device it was measured on (architecture and compute-unit count); entries from
 another device are not loaded.  No counterpart in the reference, whose
 tiling comes from fixed host heuristics (src/tiling/heuristics/).
 """
@@ -26,9 +26,12 @@ def default_path() -> str:
         os.path.expanduser("~"), ".cache", "tcuscan", "batch_tuning.json")
 
 
-def _device_key(device: int = 0) -> dict:
+def device_key(device: int = 0) -> dict:
+    """What a cache file must match to be loaded: the GPU architecture
+    (gcnArchName without its feature suffixes) and compute-unit count."""
     p = torch.cuda.get_device_properties(device)
-    return {"name": p.name, "arch": getattr(p, "gcnArchName", ""), "compute_units": p.multi_processor_count}
+    arch = getattr(p, "gcnArchName", "") or ""
+    return {"arch": arch.split(":")[0], "compute_units": int(p.multi_processor_count)}
 
 
 def save(path: str | None = None, device: int = 0) -> str:
@@ -38,7 +41,7 @@ def save(path: str | None = None, device: int = 0) -> str:
     os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
     tmp = f"{path}.tmp{os.getpid()}"
     with open(tmp, "w") as f:
-        json.dump({"version": 1, "device": _device_key(device), "entries": entries}, f, indent=1)
+        json.dump({"version": 1, "device": device_key(device), "entries": entries}, f, indent=1)
     os.replace(tmp, path)
     return path
 
@@ -53,7 +56,7 @@ def load(path: str | None = None, device: int = 0) -> int:
             d = json.load(f)
     except (OSError, ValueError):
         return 0
-    if d.get("version") != 1 or d.get("device") != _device_key(device):
+    if d.get("version") != 1 or d.get("device") != device_key(device):
         return 0
     n = 0
     for e in d.get("entries", []):

@@ -221,6 +221,38 @@ def test_seg_sum_config4_shape(ops, dev):
     assert_exact(to_np(y), oracle.seg_sum_indptr(x, indptr))
 
 
+@pytest.mark.parametrize("p_empty", [0.5, 0.99, 0.999])
+@pytest.mark.parametrize("dtype", [np.int8, np.float16], ids=["i8", "f16"])
+def test_seg_sum_tiles_with_many_rows(ops, dev, p_empty, dtype):
+    """Mostly empty rows: with p_empty = 0.99 and above a 16K-element int8
+    tile overlaps more than 65536 rows, so the kernel finds each row end's
+    row by binary search instead of its 16-bit LDS record; every tile size
+    and row pattern must give the oracle's sums."""
+    rng = np.random.default_rng(int(p_empty * 1000))
+    rows = 3_000_000
+    lens = np.where(rng.random(rows) < p_empty, 0, rng.geometric(1 / 8, rows))
+    lens[-1] = 5
+    indptr = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    nnz = int(indptr[-1])
+    x = (rng.integers(-128, 128, nnz) if dtype == np.int8 else rng.standard_normal(nnz)).astype(dtype)
+    y = ops.run_seg_sum_multi_core(torch.from_numpy(x).to(dev), torch.from_numpy(indptr).to(dev), 128)
+    assert_exact(to_np(y), oracle.seg_sum_indptr(x, indptr), f"p_empty={p_empty} nnz={nnz}")
+
+
+def test_seg_sum_int8_large_geometric(ops, dev):
+    """2^25 int8 values in geometric segments (mean 64): many 16K-element tiles
+    with rows spanning tile boundaries."""
+    rng = np.random.default_rng(25)
+    n = 1 << 25
+    lens = rng.geometric(1 / 64, n // 32)
+    ends = np.cumsum(lens)
+    ends = ends[ends < n]
+    indptr = np.concatenate([[0], ends, [n]]).astype(np.int32)
+    x = rng.integers(-128, 128, n).astype(np.int8)
+    y = ops.run_seg_sum_multi_core(torch.from_numpy(x).to(dev), torch.from_numpy(indptr).to(dev), 128)
+    assert_exact(to_np(y), oracle.seg_sum_indptr(x, indptr))
+
+
 # --- flag segmented sum (tests/test_segmented_sum.py:36-112) -------------------
 @pytest.mark.parametrize("multiplier", [1, 2, 3, 5, 6, 7, 8, 9, 10, 11, 12])
 def test_seg_sum_flags_s32(ops, dev, multiplier):
