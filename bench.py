@@ -495,12 +495,12 @@ def _copy_workload(n: int, steps: int, warmup: int):
 
 
 def _batched_workload(steps: int, warmup: int, num_scans: int = 10_000, num_segsums: int = 2_000,
-                      per_call_leg: bool = True, tune_cache: str | None = None, branches: int = 2):
+                      per_call_leg: bool = True, tune_cache: str | None = None, branches: int = 1):
     """The batched uneven workload: `num_scans` fp16 scans with lengths
     log-uniform in [2^8, 2^20], concatenated with int8 start flags, plus
     `num_segsums` indptr seg-sums over int8 data (geometric segment lengths,
     mean 64; 2^U(4, 12) segments per problem).  The batched form is one
-    BatchPlan (tuned, then captured as one HIP graph) launched per step; the
+    BatchPlan (tuned, then captured as one HIP graph, ops in sequence) launched per step; the
     per-call form launches run_scan_multi_core on every scan and
     run_seg_sum_multi_core on every seg-sum problem, each its own tensor.
     With per_call_leg False only the graph replays run (for profiling);
@@ -794,7 +794,9 @@ def main() -> int:
     ap.add_argument("--no-extras", action="store_true", help="skip the extra workloads")
     ap.add_argument("--serve-seconds", type=float, default=60.0, help="length of the serving stream")
     ap.add_argument("--serve-rate", type=float, default=2000.0, help="Poisson arrival rate, req/s")
-    ap.add_argument("--branches", type=int, default=2, help="with --only batched: graph branches of the plan")
+    ap.add_argument("--branches", type=int, default=1,
+                    help="with --only batched: graph branches of the plan (one measured faster: the persistent "
+                         "seg-scan holds every CU, so a seg-sum beside it only finishes after it)")
     ap.add_argument("--tune-cache", default=None,
                     help="with --only batched: load the batch tuner's cache from this JSON file (and save it)")
     ap.add_argument("--only", choices=["scan_f16", "scan_i8", "scan_bf16", "scan_f16_mfma", "scan_i8_mfma",
