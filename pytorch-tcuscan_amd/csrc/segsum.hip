@@ -45,6 +45,13 @@ namespace tcs {
 // any source uses, sizes the workspaces.
 constexpr int kSegNW = 8;
 constexpr int64_t kSegMinTile = 8192;
+// Tiles reduced by one workgroup, all loaded before the first is reduced.
+// Measured on MI355X (deferred carries, SpMV 2^22 x 2^22 with 6.7e7
+// lognormal-row nonzeros / fp16 seg-sum of the same rows): 1 tile 0.529 /
+// 0.100 ms, 2 tiles 0.548 / 0.122 ms, 4 tiles 0.761 / 0.124 ms; more tiles
+// per workgroup cost occupancy (SpMV: 79 VGPRs at 1, 113 at 2) and
+// serialise each workgroup's phases.
+constexpr int kSegTPW = 1;
 
 template <class IdxT>
 __device__ __forceinline__ int64_t row_end(const IdxT* __restrict__ ends, int64_t r, int64_t n_ends,
@@ -221,11 +228,34 @@ struct SrcSpmv {
   }
 };
 
-// SPLIT workgroups per tile: workgroup b reduces tile b / SPLIT over part
-// b % SPLIT of the values (see the split SpMV above), with that part's own
-// descriptors, and writes its row sums as doubles to part[p * R + row];
-// SPLIT = 1 writes y.
-template <class Src, class IdxT, class OutT, int SPLIT>
+// One tile's loaded state, per thread: the values (and for SpMV column
+// indices and gathered x), the tile's span and rows, and the row ends this
+// thread marks.
+template <class Src>
+struct SegTile {
+  typename Src::Regs regs;
+  int64_t t0, t1, r_lo, r_hi, cnt;
+  bool full, k16;
+  uint32_t e_early[2], ep_early[2];
+};
+
+// SPLIT workgroups per tile group: workgroup b handles part b % SPLIT of the
+// values (see the split SpMV above) with that part's own descriptors and
+// writes its row sums as doubles to part[p * R + row]; SPLIT = 1 writes y.
+// Each workgroup reduces TPW consecutive tiles (b / SPLIT) * TPW + m, and
+// every load of all of them (values, column indices, row ends, gathers of
+// x) is issued before the first is reduced.
+//
+// No tile waits for its look-back: a tile's rows that start inside it are
+// complete without the carry from earlier tiles, so they are written as soon
+// as the tile is reduced, and the tile publishes its aggregate at once.  At
+// most one row end per tile needs the carry (the first end, when no row
+// starts before it: the row came in from an earlier tile); its sum without
+// the carry waits in LDS.  After the last tile one walk finds the carry into
+// the first tile, the later tiles' carries follow locally from the earlier
+// tiles' aggregates, and the open ends and missing inclusive descriptors are
+// written then.
+template <class Src, class IdxT, class OutT, int SPLIT, int TPW>
 __global__ __launch_bounds__(kSegNW * kWave) void seg_reduce_kernel(
     Src src, const IdxT* __restrict__ ends, int64_t R, int64_t n_ends, int64_t nnz,
     OutT* __restrict__ y, double* __restrict__ part_out, const uint64_t* __restrict__ tile_row,
@@ -244,150 +274,263 @@ __global__ __launch_bounds__(kSegNW * kWave) void seg_reduce_kernel(
   constexpr bool kRowK16 = TILE > 8192;
   using RowK = typename std::conditional<kRowK16, uint16_t, uint32_t>::type;
   __shared__ RowK s_rowk[TILE];
-  __shared__ Acc s_wval[NW], s_wexcl[NW], s_carry;
-  __shared__ int s_wflag[NW], s_wexclf[NW];
+  __shared__ Acc s_wval[NW];
+  __shared__ int s_wflag[NW];
+  __shared__ Acc s_open_val[TPW];
+  __shared__ int64_t s_open_row[TPW];
+  __shared__ int s_open_has[TPW];
   __shared__ LaunchTag s_tag;
 
   // Every workgroup arrives, including those that return early below.
   if (threadIdx.x == 0) s_tag = launch_arrive(ws);
   const int lane = __lane_id();
   const int wid = threadIdx.x / kWave;
-  const uint32_t tile = blockIdx.x / SPLIT;
+  const uint32_t tile_base = blockIdx.x / SPLIT * (uint32_t)TPW;
   const int part = SPLIT > 1 ? (int)(blockIdx.x % SPLIT) : 0;
   src.set_part(part, SPLIT);
-  const int64_t t0 = (int64_t)tile * TILE;
-  // Loads that do not depend on one another go out together: the last row
-  // end, the tile's first rows, and the tile's values (bounded by nnz;
-  // values past the last row end belong to no row and are never stored).
   const int64_t end_all = row_end(ends, R - 1, n_ends, nnz);
-  int64_t r_lo = (int64_t)tile_row[tile];
-  int64_t r_hi = tile + 1 < num_tiles ? (int64_t)tile_row[tile + 1] : R - 1;
-  const int64_t t1 = t0 + TILE < nnz ? t0 + TILE : nnz;
-  typename Src::Regs regs;
-  const int64_t wbase = t0 + (int64_t)wid * WSEG + (int64_t)lane * G;
-  const bool full = t0 + TILE <= nnz;
-  if (full) {
-#pragma unroll
-    for (int j = 0; j < J; ++j) src.template load<true>(regs, j, wbase + j * GSPAN, t1);
-  } else {
-#pragma unroll
-    for (int j = 0; j < J; ++j) src.template load<false>(regs, j, wbase + j * GSPAN, t1);
-  }
-  if (t0 >= end_all) return;  // no row reaches this tile (nor any later one)
-
-  // Rows touching the tile: r_lo (holds t0) .. r_hi; their ends are
-  // end(r_lo - 1 + k) for k in [0, cnt).  The ends of the first
-  // 2 * NW * 64 are loaded before the gathers are issued.
-  if (r_lo > R - 1) r_lo = R - 1;
-  if (r_hi > R - 1) r_hi = R - 1;
-  if (r_hi < r_lo) r_hi = r_lo;
-  const int64_t cnt = r_hi - r_lo + 2;
-  const bool k16 = !kRowK16 || cnt <= 65536;
+  const int64_t lane_off = (int64_t)wid * WSEG + (int64_t)lane * G;
   constexpr int kEarly = 2;
-  int64_t e_early[kEarly], ep_early[kEarly];
-#pragma unroll
-  for (int q = 0; q < kEarly; ++q) {
-    const int64_t k = threadIdx.x + (int64_t)q * NW * kWave;
-    e_early[q] = k < cnt ? row_end(ends, r_lo - 1 + k, n_ends, nnz) : -1;
-    ep_early[q] = (k > 0 && k < cnt) ? row_end(ends, r_lo - 2 + k, n_ends, nnz) : -1;
-  }
-  if (full) {
-#pragma unroll
-    for (int j = 0; j < J; ++j) src.template gather<true>(regs, j, wbase + j * GSPAN, t1);
-  } else {
-#pragma unroll
-    for (int j = 0; j < J; ++j) src.template gather<false>(regs, j, wbase + j * GSPAN, t1);
-  }
 
-  // Mark the boundaries.  Among equal ends (empty rows) only the first, the
-  // row that holds the element before it, marks them, so every position is
-  // written by at most one thread.
-  for (int k = threadIdx.x; k < TILE / 32; k += NW * kWave) {
-    s_head[k] = 0;
-    s_endb[k] = 0;
-  }
-  __syncthreads();
-  auto mark = [&](int64_t k, int64_t e, int64_t ep) {
-    if (k > 0 && ep == e) return;  // an empty row: the same boundary as before
-    if (e >= t0 && e < t1) {
-      const int q = (int)(e - t0);
-      atomicOr(&s_head[q >> 5], 1u << (q & 31));
+  // Loads that do not depend on one another go out together: the last row
+  // end, each tile's first rows and its values (bounded by nnz; values past
+  // the last row end belong to no row and are never stored).
+  SegTile<Src> st[TPW];
+#pragma unroll
+  for (int m = 0; m < TPW; ++m) {
+    const uint32_t tile = tile_base + m;
+    SegTile<Src>& a = st[m];
+    a.t0 = (int64_t)tile * TILE;
+    a.t1 = a.t0 + TILE < nnz ? a.t0 + TILE : nnz;
+    a.full = a.t0 + TILE <= nnz;
+    if (tile >= num_tiles) continue;
+    a.r_lo = (int64_t)tile_row[tile];
+    a.r_hi = tile + 1 < num_tiles ? (int64_t)tile_row[tile + 1] : R - 1;
+    if (a.full) {
+#pragma unroll
+      for (int j = 0; j < J; ++j) src.template load<true>(a.regs, j, a.t0 + lane_off + j * GSPAN, a.t1);
+    } else {
+#pragma unroll
+      for (int j = 0; j < J; ++j) src.template load<false>(a.regs, j, a.t0 + lane_off + j * GSPAN, a.t1);
     }
-    if (e > t0 && e <= t1) {
-      const int q = (int)(e - 1 - t0);
-      atomicOr(&s_endb[q >> 5], 1u << (q & 31));
-      if (k16) s_rowk[q] = (RowK)k;
+  }
+  if (st[0].t0 >= end_all) return;  // no row reaches these tiles (nor any later one)
+
+  // Rows touching a tile: r_lo (holds t0) .. r_hi; their ends are
+  // end(r_lo - 1 + k) for k in [0, cnt).  Each thread's first kEarly ends
+  // (k = thread, thread + 512) are loaded before any gather is issued.
+  // Row ends are below nnz < 2^32 (checked on the host): 32-bit registers.
+#pragma unroll
+  for (int m = 0; m < TPW; ++m) {
+    SegTile<Src>& a = st[m];
+    if (tile_base + m >= num_tiles) continue;
+    if (a.r_lo > R - 1) a.r_lo = R - 1;
+    if (a.r_hi > R - 1) a.r_hi = R - 1;
+    if (a.r_hi < a.r_lo) a.r_hi = a.r_lo;
+    a.cnt = a.r_hi - a.r_lo + 2;
+    a.k16 = !kRowK16 || a.cnt <= 65536;
+#pragma unroll
+    for (int q = 0; q < kEarly; ++q) {
+      const int64_t k = threadIdx.x + (int64_t)q * NW * kWave;
+      a.e_early[q] = k < a.cnt ? (uint32_t)row_end(ends, a.r_lo - 1 + k, n_ends, nnz) : 0u;
+      a.ep_early[q] = (k > 0 && k < a.cnt) ? (uint32_t)row_end(ends, a.r_lo - 2 + k, n_ends, nnz) : 0u;
     }
-  };
-#pragma unroll
-  for (int q = 0; q < kEarly; ++q) {
-    const int64_t k = threadIdx.x + (int64_t)q * NW * kWave;
-    if (k < cnt) mark(k, e_early[q], ep_early[q]);
   }
-  for (int64_t k = threadIdx.x + (int64_t)kEarly * NW * kWave; k < cnt; k += NW * kWave)
-    mark(k, row_end(ends, r_lo - 1 + k, n_ends, nnz), row_end(ends, r_lo - 2 + k, n_ends, nnz));
-  __syncthreads();
-
-  // Heads and ends of each lane group (G = 8 elements at an 8-aligned offset).
-  uint32_t hmask[J], emask[J];
 #pragma unroll
-  for (int j = 0; j < J; ++j) {
-    const int q = (int)(wbase + j * GSPAN - t0);
-    constexpr uint32_t kGroupBits = G == 32 ? ~0u : ((1u << G) - 1u);
-    hmask[j] = (s_head[q >> 5] >> (q & 31)) & kGroupBits;
-    emask[j] = (s_endb[q >> 5] >> (q & 31)) & kGroupBits;
+  for (int m = 0; m < TPW; ++m) {
+    SegTile<Src>& a = st[m];
+    if (tile_base + m >= num_tiles) continue;
+    if (a.full) {
+#pragma unroll
+      for (int j = 0; j < J; ++j) src.template gather<true>(a.regs, j, a.t0 + lane_off + j * GSPAN, a.t1);
+    } else {
+#pragma unroll
+      for (int j = 0; j < J; ++j) src.template gather<false>(a.regs, j, a.t0 + lane_off + j * GSPAN, a.t1);
+    }
   }
 
-  // Segmented reduction of the tile (as in the scans).
-  Acc gex[J];
-  int gexf[J];
-  Acc wtot = Acc(0);
-  int wtotf = 0;
+  uint64_t* desc = ws_desc(ws) + (size_t)part * num_tiles * (kDescBytes / 8);
+  Acc agg[TPW];
+  int aggf[TPW];
+  int nv = 0;  // tiles of this group that hold row ends (block-uniform)
 #pragma unroll
-  for (int j = 0; j < J; ++j) {
-    Acc tv = Acc(0);
-    int tf = 0;
+  for (int m = 0; m < TPW; ++m) {
+    SegTile<Src>& a = st[m];
+    const uint32_t tile = tile_base + m;
+    agg[m] = Acc(0);
+    aggf[m] = 0;
+    // Block-uniform: later tiles of the group may lie past the last tile or
+    // the last row.
+    if (tile >= num_tiles || a.t0 >= end_all) break;
+    nv = m + 1;
+    const int64_t t0 = a.t0, t1 = a.t1;
+    const int64_t wbase = t0 + lane_off;
+
+    // The previous tile's outputs have read s_rowk and s_wval.
+    if (m > 0) __syncthreads();
+    // Mark the boundaries.  Among equal ends (empty rows) only the first,
+    // the row that holds the element before it, marks them, so every
+    // position is written by at most one thread.
+    for (int k = threadIdx.x; k < TILE / 32; k += NW * kWave) {
+      s_head[k] = 0;
+      s_endb[k] = 0;
+    }
+    if (threadIdx.x == 0) s_open_has[m] = 0;
+    __syncthreads();
+    auto mark = [&](int64_t k, int64_t e, int64_t ep) {
+      if (k > 0 && ep == e) return;  // an empty row: the same boundary as before
+      if (e >= t0 && e < t1) {
+        const int q = (int)(e - t0);
+        atomicOr(&s_head[q >> 5], 1u << (q & 31));
+      }
+      if (e > t0 && e <= t1) {
+        const int q = (int)(e - 1 - t0);
+        atomicOr(&s_endb[q >> 5], 1u << (q & 31));
+        if (a.k16) s_rowk[q] = (RowK)k;
+      }
+    };
 #pragma unroll
-    for (int i = 0; i < G; ++i) {
-      const Acc v = src.value(regs, j, i);
-      if (hmask[j] >> i & 1u) {
-        tv = v;
-        tf = 1;
+    for (int q = 0; q < kEarly; ++q) {
+      const int64_t k = threadIdx.x + (int64_t)q * NW * kWave;
+      if (k < a.cnt) mark(k, (int64_t)a.e_early[q], k > 0 ? (int64_t)a.ep_early[q] : -1);
+    }
+    for (int64_t k = threadIdx.x + (int64_t)kEarly * NW * kWave; k < a.cnt; k += NW * kWave)
+      mark(k, row_end(ends, a.r_lo - 1 + k, n_ends, nnz), row_end(ends, a.r_lo - 2 + k, n_ends, nnz));
+    __syncthreads();
+
+    // Heads and ends of each lane group (G elements at a G-aligned offset).
+    uint32_t hmask[J], emask[J];
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+      const int q = (int)(wbase + j * GSPAN - t0);
+      constexpr uint32_t kGroupBits = G == 32 ? ~0u : ((1u << G) - 1u);
+      hmask[j] = (s_head[q >> 5] >> (q & 31)) & kGroupBits;
+      emask[j] = (s_endb[q >> 5] >> (q & 31)) & kGroupBits;
+    }
+
+    // Segmented reduction of the tile (as in the scans).
+    Acc gex[J];
+    int gexf[J];
+    Acc wtot = Acc(0);
+    int wtotf = 0;
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+      Acc tv = Acc(0);
+      int tf = 0;
+#pragma unroll
+      for (int i = 0; i < G; ++i) {
+        const Acc v = src.value(a.regs, j, i);
+        if (hmask[j] >> i & 1u) {
+          tv = v;
+          tf = 1;
+        } else {
+          tv = tv + v;
+        }
+      }
+      int wf = tf;
+      Acc wv = tv;
+      wave_seg_incl_scan(wf, wv);
+      const Acc ev = dpp_mov<kWaveShr1>(wv);
+      const int ef = dpp_i32<kWaveShr1>(wf);
+      gex[j] = ef ? ev : wtot + ev;
+      gexf[j] = wtotf | ef;
+      const Acc gt = readlane(wv, kWave - 1);
+      const int gf = readlane(wf, kWave - 1);
+      wtot = gf ? gt : wtot + gt;
+      wtotf |= gf;
+    }
+    if (lane == 0) {
+      s_wval[wid] = wtot;
+      s_wflag[wid] = wtotf;
+    }
+    __syncthreads();
+
+    // Every wave combines the wave totals before it (its exclusive prefix
+    // within the tile) and all of them (the tile's aggregate).
+    Acc wx = Acc(0), ta = Acc(0);
+    int wxf = 0, taf = 0;
+#pragma unroll
+    for (int k = 0; k < NW; ++k) {
+      const Acc v = s_wval[k];
+      const int f = s_wflag[k];
+      if (k == wid) {
+        wx = ta;
+        wxf = taf;
+      }
+      ta = f ? v : ta + v;
+      taf |= f;
+    }
+    agg[m] = ta;
+    aggf[m] = taf;
+    if (threadIdx.x == 0) {
+      if (tile == 0) {
+        if (num_tiles > 1) desc_publish<Acc>(desc, 0, ta, s_tag.tag, kFlagIncl);
       } else {
-        tv = tv + v;
+        desc_publish<Acc>(desc, tile, ta, s_tag.tag, taf ? kFlagIncl : kFlagAgg);
       }
     }
-    int wf = tf;
-    Acc wv = tv;
-    wave_seg_incl_scan(wf, wv);
-    const Acc ev = dpp_mov<kWaveShr1>(wv);
-    const int ef = dpp_i32<kWaveShr1>(wf);
-    gex[j] = ef ? ev : wtot + ev;
-    gexf[j] = wtotf | ef;
-    const Acc gt = readlane(wv, kWave - 1);
-    const int gf = readlane(wf, kWave - 1);
-    wtot = gf ? gt : wtot + gt;
-    wtotf |= gf;
-  }
-  if (lane == 0) {
-    s_wval[wid] = wtot;
-    s_wflag[wid] = wtotf;
-  }
-  __syncthreads();
 
-  if (wid == 0) {
-    Acc a = lane < NW ? s_wval[lane] : Acc(0);
-    int af = lane < NW ? s_wflag[lane] : 0;
-    wave_seg_incl_scan(af, a);
-    const Acc agg = readlane(a, NW - 1);
-    const int aggf = readlane(af, NW - 1);
-    const Acc xa = dpp_mov<kWaveShr1>(a);
-    const int xf = dpp_i32<kWaveShr1>(af);
-    if (lane < NW) {
-      s_wexcl[lane] = xa;
-      s_wexclf[lane] = xf;
+    // Row sums.  A run with a row start before it in the tile is complete;
+    // the one run without (at most one end per tile) waits for the carry.
+    src.barrier(a.regs);
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+      if (!emask[j]) continue;
+      Acc run = gexf[j] ? gex[j] : wx + gex[j];
+      int rf = gexf[j] | wxf;
+      const int q0 = (int)(wbase + j * GSPAN - t0);
+#pragma unroll
+      for (int i = 0; i < G; ++i) {
+        const Acc v = src.value(a.regs, j, i);
+        if (hmask[j] >> i & 1u) {
+          run = v;
+          rf = 1;
+        } else {
+          run = run + v;
+        }
+        if (emask[j] >> i & 1u) {
+          int64_t row;
+          if (!kRowK16 || a.k16) {
+            row = a.r_lo - 1 + (int64_t)s_rowk[q0 + i];
+          } else {
+            // The row holding position t0 + q0 + i: the first whose end is past it.
+            int64_t lo = a.r_lo - 1 < 0 ? 0 : a.r_lo - 1, hi = a.r_hi;
+            while (lo < hi) {
+              const int64_t mid = (lo + hi) >> 1;
+              if (row_end(ends, mid, n_ends, nnz) <= t0 + q0 + i) lo = mid + 1; else hi = mid;
+            }
+            row = lo;
+          }
+          if (row < 0) row = 0;
+          if (row > R - 1) row = R - 1;
+          if (rf || tile == 0) {
+            if constexpr (SPLIT > 1) {
+              part_out[(int64_t)part * R + row] = (double)run;
+            } else {
+              y[row] = (OutT)run;
+            }
+          } else {
+            s_open_val[m] = run;
+            s_open_row[m] = row;
+            s_open_has[m] = 1;
+          }
+        }
+      }
     }
-    const bool first_head = readlane((int)(hmask[0] & 1u), 0) != 0;
+  }
+
+  // The carries: one walk for the first tile when any tile needs it, then
+  // tile by tile from the aggregates.
+  __syncthreads();
+  if (wid != 0) return;
+  bool need = false, closed = false;
+#pragma unroll
+  for (int m = 0; m < TPW; ++m) {
+    if (m < nv && !closed && (s_open_has[m] || !aggf[m])) need = true;
+    if (m < nv) closed = closed || aggf[m];
+  }
+  Acc carry = Acc(0);
+  if (need && tile_base != 0) {
     // Fallback: {row head inside tile u, sum since its last head}.
     auto fb = [&](int64_t u) {
       const int64_t u0 = u * TILE;
@@ -404,65 +547,24 @@ __global__ __launch_bounds__(kSegNW * kWave) void seg_reduce_kernel(
       for (int64_t p = from + lane; p < u1; p += kWave) sum = sum + src.value_at(p);
       return FlagVal<Acc>{s >= u0 ? 1 : 0, wave_sum(sum)};
     };
-    uint64_t* desc = ws_desc(ws) + (size_t)part * num_tiles * (kDescBytes / 8);
-    const uint32_t tag = s_tag.tag;
-    const int max_polls = s_tag.polls;
-    Acc carry = Acc(0);
-    if (tile == 0) {
-      if (lane == 0 && num_tiles > 1) desc_publish<Acc>(desc, 0, agg, tag, kFlagIncl);
-    } else if (aggf) {
-      if (lane == 0) desc_publish<Acc>(desc, tile, agg, tag, kFlagIncl);
-      if (!first_head) carry = lookback<Acc>(desc, tile, tag, max_polls, fb);
-    } else {
-      if (lane == 0) desc_publish<Acc>(desc, tile, agg, tag, kFlagAgg);
-      carry = lookback<Acc>(desc, tile, tag, max_polls, fb);
-      if (lane == 0) desc_publish<Acc>(desc, tile, carry + agg, tag, kFlagIncl);
-    }
-    if (lane == 0) s_carry = carry;
-  }
-  __syncthreads();
-
-  src.barrier(regs);
-  Acc cin = s_carry;
-  {
-    const Acc wx = s_wexcl[wid];
-    cin = s_wexclf[wid] ? wx : cin + wx;
+    carry = lookback<Acc>(desc, tile_base, s_tag.tag, s_tag.polls, fb);
   }
 #pragma unroll
-  for (int j = 0; j < J; ++j) {
-    if (!emask[j]) continue;
-    Acc run = gexf[j] ? gex[j] : cin + gex[j];
-    const int q0 = (int)(wbase + j * GSPAN - t0);
-#pragma unroll
-    for (int i = 0; i < G; ++i) {
-      const Acc v = src.value(regs, j, i);
-      if (hmask[j] >> i & 1u) {
-        run = v;
-      } else {
-        run = run + v;
-      }
-      if (emask[j] >> i & 1u) {
-        int64_t row;
-        if (!kRowK16 || k16) {
-          row = r_lo - 1 + (int64_t)s_rowk[q0 + i];
-        } else {
-          // The row holding position t0 + q0 + i: the first whose end is past it.
-          int64_t lo = r_lo - 1 < 0 ? 0 : r_lo - 1, hi = r_hi;
-          while (lo < hi) {
-            const int64_t mid = (lo + hi) >> 1;
-            if (row_end(ends, mid, n_ends, nnz) <= t0 + q0 + i) lo = mid + 1; else hi = mid;
-          }
-          row = lo;
-        }
-        if (row < 0) row = 0;
-        if (row > R - 1) row = R - 1;
+  for (int m = 0; m < TPW; ++m) {
+    if (m >= nv) break;
+    const uint32_t tile = tile_base + m;
+    if (lane == 0) {
+      if (tile != 0 && !aggf[m]) desc_publish<Acc>(desc, tile, carry + agg[m], s_tag.tag, kFlagIncl);
+      if (s_open_has[m]) {
+        const Acc v = carry + s_open_val[m];
         if constexpr (SPLIT > 1) {
-          part_out[(int64_t)part * R + row] = (double)run;
+          part_out[(int64_t)part * R + s_open_row[m]] = (double)v;
         } else {
-          y[row] = (OutT)run;
+          y[s_open_row[m]] = (OutT)v;
         }
       }
     }
+    carry = aggf[m] ? agg[m] : carry + agg[m];
   }
 }
 
@@ -529,9 +631,10 @@ static int launch_seg(const Src& src, const void* ends_v, int64_t R, int64_t n_e
                      ends, R, n_ends, nnz, y, part, tile_row, plan.num_tiles);
   TCS_CHECK_LAUNCH();
   if (plan.num_tiles == 0) return TCS_OK;
+  const uint32_t groups = (plan.num_tiles + kSegTPW - 1) / kSegTPW;
   if (plan.parts > 1) {
     if constexpr (Src::kSplittable && std::is_same<OutT, float>::value) {
-      hipLaunchKernelGGL((seg_reduce_kernel<Src, IdxT, OutT, 2>), dim3(plan.num_tiles * 2u),
+      hipLaunchKernelGGL((seg_reduce_kernel<Src, IdxT, OutT, 2, kSegTPW>), dim3(groups * 2u),
                          dim3(kSegNW * kWave), 0, st, src, ends, R, n_ends, nnz, y, part, tile_row,
                          plan.num_tiles, ws);
       TCS_CHECK_LAUNCH();
@@ -542,7 +645,7 @@ static int launch_seg(const Src& src, const void* ends_v, int64_t R, int64_t n_e
       return TCS_ERR_UNSUPPORTED_DTYPE;
     }
   }
-  hipLaunchKernelGGL((seg_reduce_kernel<Src, IdxT, OutT, 1>), dim3(plan.num_tiles), dim3(kSegNW * kWave),
+  hipLaunchKernelGGL((seg_reduce_kernel<Src, IdxT, OutT, 1, kSegTPW>), dim3(groups), dim3(kSegNW * kWave),
                      0, st, src, ends, R, n_ends, nnz, y, nullptr, tile_row, plan.num_tiles, ws);
   TCS_CHECK_LAUNCH();
   return TCS_OK;
