@@ -834,21 +834,26 @@ __global__ __launch_bounds__(kCountBlock) void compress_counted_kernel(
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   };
-  // The wave's run [base, base + wn) goes out as a scalar head up to the
-  // first 16-byte boundary, whole 16-byte vectors (one per lane per store,
-  // nontemporal), and a scalar tail: 0.533 -> 0.528 ms against scalar
-  // nontemporal stores, whose partial lines the memory system merges worse.
+  // The wave's run [base, base + wn) goes out as a head up to the first
+  // 128-byte line boundary, whole lines as 16-byte nontemporal vectors (one
+  // per lane per store), and a tail, head and tail with nontemporal element
+  // stores.  fp32 values + int32 indices at 2^28, p = 0.3: 0.529 ms with the
+  // vectors starting at the first 16-byte boundary (so the first and last
+  // line of every run's vector body were partial), 0.520 with line-aligned
+  // vectors and cached element stores, 0.502 with line-aligned vectors and
+  // nontemporal element stores.
   auto flush = [&](auto* out) {
     using OT = typename std::remove_pointer<decltype(out)>::type;
     constexpr int V = 16 / (int)sizeof(OT);
+    constexpr int L = 128 / (int)sizeof(OT);  // elements per line
     wave_sync();
-    // Elements up to the first 16-byte boundary of the output address (the
+    // Elements up to the first line boundary of the output address (the
     // outputs need only be element-aligned).
     const uintptr_t addr = reinterpret_cast<uintptr_t>(out + base);
-    const int h0 = (int)(((16u - (uint32_t)(addr & 15u)) & 15u) / sizeof(OT));
+    const int h0 = (int)(((128u - (uint32_t)(addr & 127u)) & 127u) / sizeof(OT));
     const int h = h0 < wn ? h0 : wn;
-    const int body = (wn - h) / V * V;
-    if (lane < h) out[base + lane] = (OT)stage[lane];
+    const int body = (wn - h) / L * L;
+    for (int k = lane; k < h; k += kWave) __builtin_nontemporal_store((OT)stage[k], out + base + k);
     for (int k = lane; k * V < body; k += kWave) {
       OT v[V];
 #pragma unroll
@@ -857,8 +862,7 @@ __global__ __launch_bounds__(kCountBlock) void compress_counted_kernel(
       __builtin_memcpy(&w, v, 16);
       __builtin_nontemporal_store(w, reinterpret_cast<u32x4*>(out + base + h + (int64_t)k * V));
     }
-    const int t = h + body + lane;
-    if (t < wn) out[base + t] = (OT)stage[t];
+    for (int t = h + body + lane; t < wn; t += kWave) __builtin_nontemporal_store((OT)stage[t], out + base + t);
     wave_sync();
   };
 #pragma unroll

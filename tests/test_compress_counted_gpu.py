@@ -2,8 +2,10 @@
 counts and block offsets in the workspace, and tcs_compress_counted places
 every 2048-element chunk from them (no look-back).  Checked against the
 oracle for every element size, every index mode, ragged lengths, outputs
-that start at any element-aligned address (the vector stores begin at the
-first 16-byte boundary), a capacity below the kept count, and against
+that start at any element-aligned address (each run's vector stores cover
+whole 128-byte lines; the head before the first line boundary, up to 127
+int8 elements, and the tail are stored element by element), a capacity
+below the kept count, and against
 tcs_compress (the look-back form) bit for bit."""
 import ctypes
 import os
@@ -96,14 +98,17 @@ def test_counted_gathered_indices(lib, dev, idx_dtype):
     assert np.array_equal(z.numpy(), ez) and np.array_equal(zi.numpy(), ei)
 
 
-@pytest.mark.parametrize("offset", [1, 2, 3, 5, 7])
-@pytest.mark.parametrize("dtype", [torch.float16, torch.float32], ids=str)
+@pytest.mark.parametrize("offset", [1, 2, 3, 5, 7, 33, 100])
+@pytest.mark.parametrize("dtype", [torch.int8, torch.float16, torch.float32, torch.float64], ids=str)
 def test_counted_outputs_at_any_alignment(lib, dev, offset, dtype):
-    """Outputs that start off a 16-byte boundary: the head before the first
-    boundary is written element by element; nothing outside the range is
-    touched."""
+    """Outputs that start off a line boundary: the head before the first
+    128-byte boundary (more elements than a wave has lanes for int8) is
+    written element by element; nothing outside the range is touched."""
     n = 700_001
-    x = torch.randn(n, generator=gen(offset)).to(dtype)
+    if dtype == torch.int8:
+        x = torch.randint(-128, 128, (n,), generator=gen(offset), dtype=torch.int8)
+    else:
+        x = torch.randn(n, generator=gen(offset)).to(dtype)
     m = (torch.rand(n, generator=gen(offset + 10)) < 0.5).to(torch.int8)
     z, zi, k, ox, oi = _run(lib, dev, x, m, 2, out_offset=offset)
     keep = (m == 1)
