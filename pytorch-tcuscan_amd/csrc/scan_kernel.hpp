@@ -186,8 +186,8 @@ __device__ __forceinline__ void tile_load(TileRegs<C, J>& r, const typename C::I
 // the wave total into the slot.  The last data wave to arrive scans the
 // wave totals and publishes the tile's descriptor: inclusive for tile 0 and
 // for tiles holding a segment head, aggregate otherwise.
-template <class C, int NW, int J, int SEG, int LOC>
-__device__ __forceinline__ void tile_reduce(const TileRegs<C, J>& r, typename C::Acc (&gex)[J],
+template <class C, int NW, int J, int SEG, int LOC, class GexT>
+__device__ __forceinline__ void tile_reduce(const TileRegs<C, J>& r, GexT&& gex,
                                             uint32_t& gexf, PipeSlot<C, NW>& sl,
                                             typename C::Acc start, void* ws, const ScanArgs& a,
                                             uint32_t tile, int wid, int lane) {
@@ -326,8 +326,8 @@ struct OutStage {
 };
 
 // Second pass over the tile: add the carries and store.
-template <class C, int NW, int J, int SEG, bool FULL>
-__device__ __forceinline__ void tile_output(TileRegs<C, J>& r, const typename C::Acc (&gex)[J],
+template <class C, int NW, int J, int SEG, bool FULL, class GexT>
+__device__ __forceinline__ void tile_output(TileRegs<C, J>& r, const GexT& gex,
                                             const uint32_t gexf, const PipeSlot<C, NW>& sl,
                                             typename C::Out* __restrict__ y, int64_t n,
                                             uint32_t tile, int wid, int lane,
@@ -479,8 +479,8 @@ __device__ __forceinline__ void load_any(TileRegs<C, J>& r, const typename C::In
   }
 }
 
-template <class C, int NW, int J, int SEG, int LOC>
-__device__ __forceinline__ void output_any(TileRegs<C, J>& r, const typename C::Acc (&gex)[J],
+template <class C, int NW, int J, int SEG, int LOC, class GexT>
+__device__ __forceinline__ void output_any(TileRegs<C, J>& r, const GexT& gex,
                                            const uint32_t gexf, const PipeSlot<C, NW>& sl,
                                            typename C::Out* __restrict__ y, const ScanArgs& a,
                                            uint32_t tile, int wid, int lane,
@@ -498,6 +498,15 @@ __device__ __forceinline__ void output_any(TileRegs<C, J>& r, const typename C::
     tile_output<C, NW, J, SEG, false>(r, gex, gexf, sl, y, a.n, tile, wid, lane, os);
   }
 }
+
+// One wave's group exclusives of one pipeline stage, kept in LDS: element
+// j of lane l at base[j * kWave] (base = that lane's first), so a wave
+// writes and reads 512 contiguous bytes per group.
+template <class Acc>
+struct LdsGex {
+  Acc* base;
+  __device__ __forceinline__ Acc& operator[](int j) const { return base[j * kWave]; }
+};
 
 // Persistent pipelined scan.  Workgroups draw tiles from ticket counters in
 // the workspace header, one tile ahead.  A single counter serialises every
@@ -558,13 +567,49 @@ __global__ __launch_bounds__((NW + 1) * kWave) void scan_kernel(
   if (t >= a.num_tiles) return;
 
   TileRegs<C, J> reg[2];
-  Acc gex[2][J];
+  // Each group's exclusive value from the reduce pass, kept until the
+  // output pass.  Segmented scans keep them in LDS: held in registers for
+  // both pipeline stages they are 2 * J doubles per lane, which is what made
+  // 32K-element fp16 segmented tiles spill.
+  constexpr bool kGexLds = SEG != kSegNone && LOC == kLocDpp;
+  __shared__ Acc gex_sh[kGexLds ? 2 * NW * J * kWave : 1];
+  Acc gex[2][kGexLds ? 1 : J];
   uint32_t gexf[2];  // bit j: group j's exclusive value starts a segment
   uint16_t* stage = stage_all[lb_wave ? 0 : wid];
   uint32_t* ostage = ostage_all[lb_wave ? 0 : wid];
+  // The parity is a compile-time constant (std::integral_constant), so
+  // every index into the two-stage arrays stays static: a runtime index
+  // would put them in scratch memory.
+  // The parity is a compile-time constant (std::integral_constant), so
+  // every index into the two-stage arrays stays static: a runtime index
+  // would put them in scratch memory.
+  auto gex_of = [&](auto parity) {
+    constexpr int par = decltype(parity)::value;
+    if constexpr (kGexLds) {
+      return LdsGex<Acc>{&gex_sh[((par * NW + (lb_wave ? 0 : wid)) * J) * kWave + lane]};
+    } else {
+      return gex[par];
+    }
+  };
+  auto reduce_into = [&](auto parity, uint32_t tile) {
+    constexpr int par = decltype(parity)::value;
+    if constexpr (kGexLds) {
+      tile_reduce<C, NW, J, SEG, LOC>(reg[par], gex_of(parity), gexf[par], slot[par], start, ws, a, tile, wid, lane);
+    } else {
+      tile_reduce<C, NW, J, SEG, LOC>(reg[par], gex[par], gexf[par], slot[par], start, ws, a, tile, wid, lane);
+    }
+  };
+  auto output_from = [&](auto parity, uint32_t tile) {
+    constexpr int par = decltype(parity)::value;
+    if constexpr (kGexLds) {
+      output_any<C, NW, J, SEG, LOC>(reg[par], gex_of(parity), gexf[par], slot[par], y, a, tile, wid, lane, stage, ostage);
+    } else {
+      output_any<C, NW, J, SEG, LOC>(reg[par], gex[par], gexf[par], slot[par], y, a, tile, wid, lane, stage, ostage);
+    }
+  };
   if (!lb_wave) {
     load_any<C, NW, J, SEG, LOC>(reg[0], x, flags, a, t, wid, lane);
-    tile_reduce<C, NW, J, SEG, LOC>(reg[0], gex[0], gexf[0], slot[0], start, ws, a, t, wid, lane);
+    reduce_into(std::integral_constant<int, 0>{}, t);
   }
   __syncthreads();
 
@@ -578,10 +623,10 @@ __global__ __launch_bounds__((NW + 1) * kWave) void scan_kernel(
       if (lane == 0) s_draw[P] = tn < a.num_tiles ? draw() : a.num_tiles;
     } else if (tn < a.num_tiles) {
       load_any<C, NW, J, SEG, LOC>(reg[Q], x, flags, a, tn, wid, lane);
-      tile_reduce<C, NW, J, SEG, LOC>(reg[Q], gex[Q], gexf[Q], slot[Q], start, ws, a, tn, wid, lane);
+      reduce_into(std::integral_constant<int, Q>{}, tn);
     }
     __syncthreads();
-    if (!lb_wave) output_any<C, NW, J, SEG, LOC>(reg[P], gex[P], gexf[P], slot[P], y, a, t, wid, lane, stage, ostage);
+    if (!lb_wave) output_from(std::integral_constant<int, P>{}, t);
     t = tn;
     tn = s_draw[P];
     return t < a.num_tiles;
