@@ -330,7 +330,8 @@ enum tcs_cmp {
 /* *count_out (device int64) = number of i with (x[i] cmp pivot), compared
  * in x's type after rounding pivot to float and then to that type; a NaN
  * counts only for TCS_CMP_NE.  x: F16, BF16, F32.  Uses
- * tcs_compress_workspace_bytes(n).  Replaces count_if_fp16 behind
+ * tcs_compress_workspace_bytes(n), where it also leaves the chunk counts
+ * that tcs_filter_counted reads.  Replaces count_if_fp16 behind
  * run_count_if (src/torch/torch_compare.h:33-67, src/compare.cpp:25-39) and
  * the mask + count steps of run_filter_greater_eq / run_filter_less_eq /
  * run_where (src/torch/torch_compress.h:319-398). */
@@ -346,6 +347,17 @@ int tcs_count_filter(const void* x, int x_dtype, double pivot, int cmp, int64_t 
  * where_fp16, src/kernels/kernel_where.h) (src/torch/torch_compress.h:319-398). */
 int tcs_filter(const void* x, int x_dtype, double pivot, int cmp, void* out_vals, int32_t* out_pos,
                int64_t n, int64_t out_capacity, void* workspace, size_t workspace_bytes, void* stream);
+
+/* tcs_filter after tcs_count_filter: identical results, computed from the
+ * kept count of every 2048-element chunk that tcs_count_filter (same x,
+ * pivot, cmp, n) left in `workspace` just before on the same stream; each
+ * chunk is compacted on its own (no look-back).  cmp: TCS_CMP_GE or
+ * TCS_CMP_LE.  Replaces the compaction step of run_filter_greater_eq,
+ * run_filter_less_eq and run_where (src/torch/torch_compress.h:319-398),
+ * whose wrappers count first and size their outputs from the count. */
+int tcs_filter_counted(const void* x, int x_dtype, double pivot, int cmp, void* out_vals, int32_t* out_pos,
+                       int64_t n, int64_t out_capacity, void* workspace, size_t workspace_bytes,
+                       void* stream);
 
 /* Segmented scan from plain scans: z[i] = scan_x[i] - seg_ends[scan_f[i] - 1]
  * (z[i] = scan_x[i] where scan_f[i] == 0), one float32 subtraction each;

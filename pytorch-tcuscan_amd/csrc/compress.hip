@@ -502,100 +502,18 @@ __global__ __launch_bounds__((kCmpNW + 1) * kWave) void filter_kernel(
   }
 }
 
-// Count of mask == 1: per-workgroup partial counts, then one workgroup sums
-// them in a fixed order.
+// Count passes: 256-thread workgroups over 64K-element blocks.
 constexpr int kCountBlock = 256;
 constexpr int64_t kCountChunk = 1 << 16;
 
-template <class XT, class Keep>
-__global__ __launch_bounds__(kCountBlock) void count_partials_kernel(const XT* __restrict__ x, Keep keep,
-                                                                     int64_t n,
-                                                                     int64_t* __restrict__ partial) {
-  const int64_t c0 = (int64_t)blockIdx.x * kCountChunk;
-  const int64_t c1 = c0 + kCountChunk < n ? c0 + kCountChunk : n;
-  // Each wave counts its kept items with ballots (one compare per item, the
-  // popcount on the scalar unit); a full chunk keeps four 16-byte loads in
-  // flight per thread and needs no bounds checks.
-  using LT = typename std::conditional<Keep::kFromX, XT, int8_t>::type;
-  constexpr int V = (int)(16 / sizeof(LT));  // items per 16-byte load
-  constexpr int U = 4;
-  constexpr int64_t STEP = (int64_t)kCountBlock * V;
-  static_assert(kCountChunk % (U * STEP) == 0, "full chunks are whole unrolled steps");
-  const LT* src;
-  if constexpr (Keep::kFromX) {
-    src = x;
-  } else {
-    src = keep.mask;
-  }
-  auto kept = [&](LT v) -> bool {
-    if constexpr (Keep::kFromX) {
-      return keep.of(v);
-    } else {
-      return v == 1;
-    }
-  };
-  uint32_t c = 0;  // the same in every lane of a wave
-  int64_t b = c0 + (int64_t)threadIdx.x * V;
-  if (c1 - c0 == kCountChunk) {
-    for (; b < c1; b += U * STEP) {
-      LT v[U][V];
-#pragma unroll
-      for (int u = 0; u < U; ++u) load_items<LT, V, true>(src, b + u * STEP, c1, v[u]);
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-#pragma unroll
-        for (int i = 0; i < V; ++i) c += (uint32_t)__popcll(__ballot(kept(v[u][i])));
-      }
-    }
-  } else {
-    // The trip count is the same for every lane (lanes past c1 load zeros).
-    for (int64_t b0 = c0; b0 < c1; b0 += STEP, b += STEP) {
-      LT v[V];
-      load_items<LT, V, false>(src, b, c1, v);
-#pragma unroll
-      for (int i = 0; i < V; ++i) c += (uint32_t)__popcll(__ballot(b + i < c1 && kept(v[i])));
-    }
-  }
-  __shared__ uint32_t s[kCountBlock / kWave];
-  if (__lane_id() == 0) s[threadIdx.x / kWave] = c;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int64_t t = 0;
-    for (int k = 0; k < kCountBlock / kWave; ++k) t += s[k];
-    partial[blockIdx.x] = t;
-  }
-}
-
-__global__ __launch_bounds__(kCountBlock) void count_final_kernel(const int64_t* __restrict__ partial,
-                                                                  int64_t np, int64_t* __restrict__ out) {
-  // Four partials in flight per thread; fixed summation order.
-  int64_t t = 0;
-  int64_t k = threadIdx.x;
-  for (; k + 3 * kCountBlock < np; k += 4 * kCountBlock) {
-    const int64_t a = partial[k], b = partial[k + kCountBlock], c = partial[k + 2 * kCountBlock],
-                  d = partial[k + 3 * kCountBlock];
-    t += (a + b) + (c + d);
-  }
-  for (; k < np; k += kCountBlock) t += partial[k];
-  __shared__ int64_t s[kCountBlock / kWave];
-  const int64_t w = wave_sum(t);
-  if (__lane_id() == 0) s[threadIdx.x / kWave] = w;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int64_t a = 0;
-#pragma unroll
-    for (int q = 0; q < kCountBlock / kWave; ++q) a += s[q];
-    out[0] = a;
-  }
-}
-
 // ---------------------------------------------------------------------------
-// Counted mask compaction.  The count pass of a mask compaction
-// (tcs_count_mask) also leaves in the workspace the kept count of every
-// 2048-element chunk and the exclusive offset of every 64K-element block,
-// so the compaction that follows (tcs_compress_counted) knows where each
-// chunk's output starts without a look-back: every wave compacts one chunk
-// on its own, with no block barrier and no waiting on other tiles.
+// Counted compaction.  The count pass of a mask compaction or a filter
+// (tcs_count_mask, tcs_count_filter) also leaves in the workspace the kept
+// count of every 2048-element chunk and the exclusive offset of every
+// 64K-element block, so the compaction that follows (tcs_compress_counted,
+// tcs_filter_counted) knows where each chunk's output starts without a
+// look-back: every wave compacts one chunk on its own, with no block
+// barrier and no waiting on other tiles.
 //
 // Workspace after the header: partial[np] (block counts, int64),
 // block_off[np] (exclusive block offsets, int64), chunk_cnt[nch] (chunk
@@ -632,7 +550,10 @@ __device__ __forceinline__ uint32_t ones_in_word(uint32_t v) {
 
 // Block b (256 threads, 64K elements): wave w counts chunks w, w + 4, ...,
 // each lane 32 mask bytes of a chunk, four chunks' loads in flight at a
-// time (the mask is read once: nontemporal loads).
+// time.  Plain loads: the compaction that follows reads the mask again and
+// finds part of it in the caches (count + compaction of fp32 values with
+// int32 indices at 2^28: 0.584 ms with nontemporal mask loads in both
+// passes, 0.547 ms with plain ones).
 __global__ __launch_bounds__(kCountBlock) void count_mask_chunks_kernel(const int8_t* __restrict__ mask,
                                                                         int64_t n, int64_t* __restrict__ partial,
                                                                         uint64_t* __restrict__ chunk_cnt) {
@@ -652,8 +573,8 @@ __global__ __launch_bounds__(kCountBlock) void count_mask_chunks_kernel(const in
       const int64_t base = b0 + (int64_t)(wid + kWaves * (i0 + i)) * kChunk + lane * 32;
       if (base + 32 <= n) {
         const u32x4* p = reinterpret_cast<const u32x4*>(mask + base);
-        v[i][0] = __builtin_nontemporal_load(p);
-        v[i][1] = __builtin_nontemporal_load(p + 1);
+        v[i][0] = p[0];
+        v[i][1] = p[1];
       } else {
         uint32_t w[8];
 #pragma unroll
@@ -684,6 +605,62 @@ __global__ __launch_bounds__(kCountBlock) void count_mask_chunks_kernel(const in
   }
   if (lane < kPerWave) {
     // Chunks past n are not written and never read.
+    const int64_t ch = (int64_t)blockIdx.x * kChunksPerBlock + wid + kWaves * lane;
+    if (ch * kChunk < n) chunk_cnt[ch] = mine;
+  }
+  __shared__ uint32_t s[kWaves];
+  if (lane == 0) s[wid] = tot;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int64_t t = 0;
+#pragma unroll
+    for (int k = 0; k < kWaves; ++k) t += s[k];
+    partial[blockIdx.x] = t;
+  }
+}
+
+// The chunk counts of a comparison (x cmp pivot) in the same layout: block
+// b (256 threads, 64K elements), wave w counts chunks w, w + 4, ..., each
+// lane 32 consecutive elements of a chunk; two chunks' loads in flight at
+// a time (plain loads, for the compaction's second read of x).  Elements
+// past n are not counted.
+template <class XT, class Keep>
+__global__ __launch_bounds__(kCountBlock) void count_keep_chunks_kernel(const XT* __restrict__ x, Keep keep,
+                                                                        int64_t n, int64_t* __restrict__ partial,
+                                                                        uint64_t* __restrict__ chunk_cnt) {
+  const int lane = __lane_id();
+  const int wid = threadIdx.x / kWave;
+  constexpr int kWaves = kCountBlock / kWave;
+  constexpr int kPerWave = kChunksPerBlock / kWaves;  // 8
+  constexpr int kBatch = 2;
+  constexpr int kItems = (int)(kChunk / kWave);  // 32 per lane
+  const int64_t b0 = (int64_t)blockIdx.x * kCountChunk;
+  uint32_t mine = 0;  // lane i keeps the count of chunk wid + 4i
+  uint32_t tot = 0;
+#pragma unroll
+  for (int i0 = 0; i0 < kPerWave; i0 += kBatch) {
+    Packed<XT, kItems> v[kBatch];
+    int64_t base[kBatch];
+#pragma unroll
+    for (int i = 0; i < kBatch; ++i) {
+      base[i] = b0 + (int64_t)(wid + kWaves * (i0 + i)) * kChunk + lane * kItems;
+      if (base[i] + kItems <= n) {
+        load_items<XT, kItems, true>(x, base[i], n, v[i]);
+      } else {
+        load_items<XT, kItems, false>(x, base[i], n, v[i]);
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < kBatch; ++i) {
+      uint32_t k = 0;
+#pragma unroll
+      for (int q = 0; q < kItems; ++q) k += (keep.of(v[i][q]) && base[i] + q < n) ? 1u : 0u;
+      const uint32_t c = (uint32_t)wave_sum((int)k);
+      tot += c;
+      mine = lane == i0 + i ? c : mine;
+    }
+  }
+  if (lane < kPerWave) {
     const int64_t ch = (int64_t)blockIdx.x * kChunksPerBlock + wid + kWaves * lane;
     if (ch * kChunk < n) chunk_cnt[ch] = mine;
   }
@@ -765,18 +742,20 @@ __device__ __forceinline__ void load_nt(const T* __restrict__ p, int64_t base, P
   }
 }
 
-// Wave-per-chunk compaction using the counts left by the count pass.
-template <int XB, int IB, int IDX>
+// Wave-per-chunk compaction using the counts left by the count pass.  Keep
+// is KeepMask (the mask array) or KeepCmp (x compared with a pivot, in
+// registers: the filters).
+template <int XB, int IB, int IDX, class Keep>
 __global__ __launch_bounds__(kCountBlock) void compress_counted_kernel(
     const typename Bits<XB>::T* __restrict__ x, const typename Bits<IB>::T* __restrict__ idx_in,
-    const int8_t* __restrict__ mask, typename Bits<XB>::T* __restrict__ out_x,
+    Keep keep, typename Bits<XB>::T* __restrict__ out_x,
     typename Bits<IB>::T* __restrict__ out_idx, int64_t n, int64_t cap, const int64_t* __restrict__ block_off,
     const uint64_t* __restrict__ chunk_cnt) {
   using XT = typename Bits<XB>::T;
   using IT = typename Bits<IB>::T;
   constexpr int G = 8, J = (int)(kChunk / (kWave * G));  // 4 groups of 8 per lane
   constexpr int kWaves = kCountBlock / kWave;
-  constexpr int kStageBytes = IDX == 0 ? XB : (XB > IB ? XB : IB);
+  constexpr int kStageBytes = IDX == 0 ? XB : IDX == 4 ? IB : (XB > IB ? XB : IB);
   using ST = typename Bits<kStageBytes>::T;
   __shared__ ST s_stage[kWaves][kChunk];
   const int lane = __lane_id();
@@ -801,19 +780,30 @@ __global__ __launch_bounds__(kCountBlock) void compress_counted_kernel(
     const int64_t gb = c0 + j * (kWave * G) + lane * G;
     Packed<int8_t, G> m;
     if (full) {
-      // Every input is read once: nontemporal loads (0.573 -> 0.533 ms for
-      // fp32 values + int32 indices at 2^28).
-      load_nt(mask, gb, m);
-      load_nt(x, gb, xv[j]);
+      // Values and indices that only this pass reads: nontemporal loads
+      // (0.573 -> 0.533 ms for fp32 values + int32 indices at 2^28).  The
+      // mask, and a filter's x, were read by the count pass just before:
+      // plain loads, which find part of them in the caches.
+      if constexpr (!Keep::kFromX) load_items<int8_t, G, true>(keep.mask, gb, n, m);
+      if constexpr (Keep::kFromX) {
+        load_items<XT, G, true>(x, gb, n, xv[j]);
+      } else {
+        load_nt(x, gb, xv[j]);
+      }
       if constexpr (IDX == 1) load_nt(idx_in, gb, iv[j]);
     } else {
-      load_items<int8_t, G, false>(mask, gb, n, m);
+      if constexpr (!Keep::kFromX) load_items<int8_t, G, false>(keep.mask, gb, n, m);
       load_items<XT, G, false>(x, gb, n, xv[j]);
       if constexpr (IDX == 1) load_items<IT, G, false>(idx_in, gb, n, iv[j]);
     }
     uint32_t b = 0;
+    if constexpr (Keep::kFromX) {
 #pragma unroll
-    for (int i = 0; i < G; ++i) b |= (m[i] == 1 ? 1u : 0u) << i;  // items past n load as 0
+      for (int i = 0; i < G; ++i) b |= (keep.of(xv[j][i]) && (full || gb + i < n) ? 1u : 0u) << i;
+    } else {
+#pragma unroll
+      for (int i = 0; i < G; ++i) b |= (m[i] == 1 ? 1u : 0u) << i;  // items past n load as 0
+    }
     mb[j] = b;
   }
   const int64_t base = wave_sum(part);
@@ -865,14 +855,16 @@ __global__ __launch_bounds__(kCountBlock) void compress_counted_kernel(
     for (int t = h + body + lane; t < wn; t += kWave) __builtin_nontemporal_store((OT)stage[t], out + base + t);
     wave_sync();
   };
+  if constexpr (IDX != 4) {
 #pragma unroll
-  for (int j = 0; j < J; ++j) {
-    int pos = gex[j];
+    for (int j = 0; j < J; ++j) {
+      int pos = gex[j];
 #pragma unroll
-    for (int i = 0; i < G; ++i)
-      if ((mb[j] >> i) & 1u) stage[pos++] = (ST)xv[j][i];
+      for (int i = 0; i < G; ++i)
+        if ((mb[j] >> i) & 1u) stage[pos++] = (ST)xv[j][i];
+    }
+    flush(out_x);
   }
-  flush(out_x);
   if constexpr (IDX != 0) {
 #pragma unroll
     for (int j = 0; j < J; ++j) {
@@ -882,7 +874,7 @@ __global__ __launch_bounds__(kCountBlock) void compress_counted_kernel(
       for (int i = 0; i < G; ++i) {
         if ((mb[j] >> i) & 1u) {
           if constexpr (IDX == 1) stage[pos] = (ST)iv[j][i];
-          if constexpr (IDX == 2) stage[pos] = (ST)(IT)(gb + i);
+          if constexpr (IDX == 2 || IDX == 4) stage[pos] = (ST)(IT)(gb + i);
           if constexpr (IDX == 3) stage[pos] = (ST)(IT)(gb + i + 1);
           ++pos;
         }
@@ -892,8 +884,8 @@ __global__ __launch_bounds__(kCountBlock) void compress_counted_kernel(
   }
 }
 
-template <int XB, int IB, int IDX>
-static int launch_counted(const void* x, const void* idx_in, const int8_t* mask, void* out_x, void* out_idx,
+template <int XB, int IB, int IDX, class Keep>
+static int launch_counted(const void* x, const void* idx_in, Keep keep, void* out_x, void* out_idx,
                           int64_t n, int64_t cap, void* ws, hipStream_t st) {
   using XT = typename Bits<XB>::T;
   using IT = typename Bits<IB>::T;
@@ -901,8 +893,8 @@ static int launch_counted(const void* x, const void* idx_in, const int8_t* mask,
   const int64_t nch = (n + kChunk - 1) / kChunk;
   const int64_t blocks = (nch + kCountBlock / kWave - 1) / (kCountBlock / kWave);
   char* w = static_cast<char*>(ws);
-  hipLaunchKernelGGL((compress_counted_kernel<XB, IB, IDX>), dim3((uint32_t)blocks), dim3(kCountBlock), 0, st,
-                     static_cast<const XT*>(x), static_cast<const IT*>(idx_in), mask, static_cast<XT*>(out_x),
+  hipLaunchKernelGGL((compress_counted_kernel<XB, IB, IDX, Keep>), dim3((uint32_t)blocks), dim3(kCountBlock), 0, st,
+                     static_cast<const XT*>(x), static_cast<const IT*>(idx_in), keep, static_cast<XT*>(out_x),
                      static_cast<IT*>(out_idx), n, cap, reinterpret_cast<const int64_t*>(w + l.block_off),
                      reinterpret_cast<const uint64_t*>(w + l.chunk_cnt));
   TCS_CHECK_LAUNCH();
@@ -912,11 +904,12 @@ static int launch_counted(const void* x, const void* idx_in, const int8_t* mask,
 template <int XB>
 static int counted_idx(int idx_mode, int idx_bytes, const void* x, const void* idx_in, const int8_t* mask,
                        void* out_x, void* out_idx, int64_t n, int64_t cap, void* ws, hipStream_t st) {
-  if (idx_mode == 0) return launch_counted<XB, 4, 0>(x, nullptr, mask, out_x, nullptr, n, cap, ws, st);
-  if (idx_mode == 2) return launch_counted<XB, 4, 2>(x, nullptr, mask, out_x, out_idx, n, cap, ws, st);
-  if (idx_mode == 3) return launch_counted<XB, 4, 3>(x, nullptr, mask, out_x, out_idx, n, cap, ws, st);
-  if (idx_bytes == 4) return launch_counted<XB, 4, 1>(x, idx_in, mask, out_x, out_idx, n, cap, ws, st);
-  if (idx_bytes == 8) return launch_counted<XB, 8, 1>(x, idx_in, mask, out_x, out_idx, n, cap, ws, st);
+  const KeepMask keep{mask};
+  if (idx_mode == 0) return launch_counted<XB, 4, 0>(x, nullptr, keep, out_x, nullptr, n, cap, ws, st);
+  if (idx_mode == 2) return launch_counted<XB, 4, 2>(x, nullptr, keep, out_x, out_idx, n, cap, ws, st);
+  if (idx_mode == 3) return launch_counted<XB, 4, 3>(x, nullptr, keep, out_x, out_idx, n, cap, ws, st);
+  if (idx_bytes == 4) return launch_counted<XB, 4, 1>(x, idx_in, keep, out_x, out_idx, n, cap, ws, st);
+  if (idx_bytes == 8) return launch_counted<XB, 8, 1>(x, idx_in, keep, out_x, out_idx, n, cap, ws, st);
   return TCS_ERR_UNSUPPORTED_DTYPE;
 }
 
@@ -1003,22 +996,6 @@ static int launch_compress(const void* x, const void* idx_in, Keep keep, void* o
   return TCS_OK;
 }
 
-// Count pass of either kind of compaction into *count_out.
-template <class XT, class Keep>
-static int launch_count(const XT* x, Keep keep, int64_t n, int64_t* count_out, void* ws, hipStream_t st) {
-  const int64_t np = (n + kCountChunk - 1) / kCountChunk;
-  // Partials after the reserved header.
-  auto* partial = reinterpret_cast<int64_t*>(static_cast<char*>(ws) + kHeaderBytes);
-  if (np > 0) {
-    hipLaunchKernelGGL((count_partials_kernel<XT, Keep>), dim3((uint32_t)np), dim3(kCountBlock), 0, st, x,
-                       keep, n, partial);
-    TCS_CHECK_LAUNCH();
-  }
-  hipLaunchKernelGGL(count_final_kernel, dim3(1), dim3(kCountBlock), 0, st, partial, np, count_out);
-  TCS_CHECK_LAUNCH();
-  return TCS_OK;
-}
-
 // Pivot rounded to the input type (through float first, as a float pivot
 // argument is), so the device compares in that type.
 static float pivot_in_type(double pivot, int dt) {
@@ -1042,10 +1019,36 @@ static int filter_launch(const void* x, float pv, void* out_vals, int32_t* out_p
   return launch_compress<XB, 4, 2>(x, nullptr, keep, out_vals, out_pos, n, cap, ws, tiles, st);
 }
 
+// Count pass of a filter: the kept count, and the chunk counts and block
+// offsets of the counted layout for tcs_filter_counted.
 template <int DT, int CMP>
 static int filter_count(const void* x, float pv, int64_t n, int64_t* count_out, void* ws, hipStream_t st) {
   using XT = typename Bits<DT == TCS_F32 ? 4 : 2>::T;
-  return launch_count(static_cast<const XT*>(x), KeepCmp<DT, CMP>{pv}, n, count_out, ws, st);
+  const CountedLayout l = counted_layout(n);
+  const int64_t np = (n + kCountChunk - 1) / kCountChunk;
+  char* w = static_cast<char*>(ws);
+  if (np > 0) {
+    hipLaunchKernelGGL((count_keep_chunks_kernel<XT, KeepCmp<DT, CMP>>), dim3((uint32_t)np), dim3(kCountBlock), 0,
+                       st, static_cast<const XT*>(x), KeepCmp<DT, CMP>{pv}, n,
+                       reinterpret_cast<int64_t*>(w + l.partial), reinterpret_cast<uint64_t*>(w + l.chunk_cnt));
+    TCS_CHECK_LAUNCH();
+  }
+  hipLaunchKernelGGL(count_scan_kernel, dim3(1), dim3(kScanBlock), 0, st,
+                     reinterpret_cast<const int64_t*>(w + l.partial), np,
+                     reinterpret_cast<int64_t*>(w + l.block_off), count_out);
+  TCS_CHECK_LAUNCH();
+  return TCS_OK;
+}
+
+// Counted filter: values (IDX 0), positions (IDX 4) or both (IDX 2).
+template <int DT, int CMP>
+static int filter_counted_launch(const void* x, float pv, void* out_vals, int32_t* out_pos, int64_t n, int64_t cap,
+                                 void* ws, hipStream_t st) {
+  constexpr int XB = DT == TCS_F32 ? 4 : 2;
+  const KeepCmp<DT, CMP> keep{pv};
+  if (!out_pos) return launch_counted<XB, 4, 0>(x, nullptr, keep, out_vals, nullptr, n, cap, ws, st);
+  if (!out_vals) return launch_counted<XB, 4, 4>(x, nullptr, keep, nullptr, out_pos, n, cap, ws, st);
+  return launch_counted<XB, 4, 2>(x, nullptr, keep, out_vals, out_pos, n, cap, ws, st);
 }
 
 template <int DT>
@@ -1138,8 +1141,7 @@ int tcs_count_filter(const void* x, int x_dtype, double pivot, int cmp, int64_t 
   if (n < 0 || !count_out || cmp < TCS_CMP_LT || cmp > TCS_CMP_NE || (n > 0 && !x)) return TCS_ERR_INVALID_ARG;
   if (n > 0 && !aligned16(x)) return TCS_ERR_INVALID_ARG;
   if (x_dtype != TCS_F16 && x_dtype != TCS_BF16 && x_dtype != TCS_F32) return TCS_ERR_UNSUPPORTED_DTYPE;
-  const int64_t np = (n + kCountChunk - 1) / kCountChunk;
-  if (!workspace || workspace_bytes < kHeaderBytes + (size_t)np * 8) return TCS_ERR_WORKSPACE;
+  if (!workspace || workspace_bytes < counted_layout(n).end) return TCS_ERR_WORKSPACE;
   hipStream_t st = static_cast<hipStream_t>(stream);
   const float pv = pivot_in_type(pivot, x_dtype);
   switch (x_dtype) {
@@ -1173,6 +1175,31 @@ int tcs_filter(const void* x, int x_dtype, double pivot, int cmp, void* out_vals
     default:
       return ge ? filter_launch<TCS_F32, TCS_CMP_GE>(x, pv, out_vals, out_pos, n, out_capacity, workspace, nt, st)
                 : filter_launch<TCS_F32, TCS_CMP_LE>(x, pv, out_vals, out_pos, n, out_capacity, workspace, nt, st);
+  }
+}
+
+int tcs_filter_counted(const void* x, int x_dtype, double pivot, int cmp, void* out_vals, int32_t* out_pos,
+                       int64_t n, int64_t out_capacity, void* workspace, size_t workspace_bytes, void* stream) {
+  using namespace tcs;
+  if (n < 0 || out_capacity < 0 || (cmp != TCS_CMP_GE && cmp != TCS_CMP_LE)) return TCS_ERR_INVALID_ARG;
+  if (x_dtype != TCS_F16 && x_dtype != TCS_BF16 && x_dtype != TCS_F32) return TCS_ERR_UNSUPPORTED_DTYPE;
+  if (n == 0) return TCS_OK;
+  if (!x || (!out_vals && !out_pos) || !aligned16(x)) return TCS_ERR_INVALID_ARG;
+  if (n >= ((int64_t)1 << 31)) return TCS_ERR_INVALID_ARG;
+  if (!workspace || workspace_bytes < counted_layout(n).end) return TCS_ERR_WORKSPACE;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const float pv = pivot_in_type(pivot, x_dtype);
+  const bool ge = cmp == TCS_CMP_GE;
+  switch (x_dtype) {
+    case TCS_F16:
+      return ge ? filter_counted_launch<TCS_F16, TCS_CMP_GE>(x, pv, out_vals, out_pos, n, out_capacity, workspace, st)
+                : filter_counted_launch<TCS_F16, TCS_CMP_LE>(x, pv, out_vals, out_pos, n, out_capacity, workspace, st);
+    case TCS_BF16:
+      return ge ? filter_counted_launch<TCS_BF16, TCS_CMP_GE>(x, pv, out_vals, out_pos, n, out_capacity, workspace, st)
+                : filter_counted_launch<TCS_BF16, TCS_CMP_LE>(x, pv, out_vals, out_pos, n, out_capacity, workspace, st);
+    default:
+      return ge ? filter_counted_launch<TCS_F32, TCS_CMP_GE>(x, pv, out_vals, out_pos, n, out_capacity, workspace, st)
+                : filter_counted_launch<TCS_F32, TCS_CMP_LE>(x, pv, out_vals, out_pos, n, out_capacity, workspace, st);
   }
 }
 

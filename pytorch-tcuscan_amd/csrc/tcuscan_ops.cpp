@@ -997,10 +997,14 @@ at::Tensor filter_impl(const at::Tensor& x_in, double pivot, int cmp, bool value
           op);
   const int64_t k = cnt.item<int64_t>();
   at::Tensor out = values ? at::empty({k}, x.options()) : at::empty({k}, x.options().dtype(at::kInt));
-  if (k > 0)
-    check(tcs_filter(x.data_ptr(), dtype_code(t), pivot, cmp, values ? out.data_ptr() : nullptr,
-                     values ? nullptr : out.data_ptr<int32_t>(), n, k, ws.first, ws.second, stream_of(x)),
+  if (k > 0) {
+    // The count pass left the chunk counts in this workspace.
+    auto wk = workspace(x, tcs_compress_workspace_bytes(n), true);
+    check(tcs_filter_counted(x.data_ptr(), dtype_code(t), pivot, cmp, values ? out.data_ptr() : nullptr,
+                             values ? nullptr : out.data_ptr<int32_t>(), n, k, wk.first, wk.second,
+                             stream_of(x)),
           op);
+  }
   return out;
 }
 
