@@ -886,7 +886,7 @@ def main() -> int:
             "value": round(fl["gbs_wall"], 2), "unit": "GB/s",
             "bytes_per_step": fl["bytes_per_step"], "kept": fl["kept"],
             "ms_per_step": round(fl["wall_s"] * 1e3 / ksteps, 4),
-            "note": "count pass + fused compare-and-compact pass, with the count read back to size the output",
+            "note": "count pass (chunk counts) + counted compare-and-compact pass, with the count read back to size the output",
         }
         _progress("radix sort")
         so = _sort_workload(n, "fp16", max(5, ksteps // 2), 2)
