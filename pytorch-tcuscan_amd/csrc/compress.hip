@@ -640,21 +640,28 @@ __global__ __launch_bounds__(kCountBlock) void count_keep_chunks_kernel(const XT
 #pragma unroll
   for (int i0 = 0; i0 < kPerWave; i0 += kBatch) {
     Packed<XT, kItems> v[kBatch];
-    int64_t base[kBatch];
+    int lim[kBatch];  // this lane's items below n (kItems except at the tail)
 #pragma unroll
     for (int i = 0; i < kBatch; ++i) {
-      base[i] = b0 + (int64_t)(wid + kWaves * (i0 + i)) * kChunk + lane * kItems;
-      if (base[i] + kItems <= n) {
-        load_items<XT, kItems, true>(x, base[i], n, v[i]);
+      const int64_t base = b0 + (int64_t)(wid + kWaves * (i0 + i)) * kChunk + lane * kItems;
+      if (base + kItems <= n) {
+        load_items<XT, kItems, true>(x, base, n, v[i]);
+        lim[i] = kItems;
       } else {
-        load_items<XT, kItems, false>(x, base[i], n, v[i]);
+        load_items<XT, kItems, false>(x, base, n, v[i]);
+        lim[i] = base >= n ? 0 : (int)(n - base);
       }
     }
 #pragma unroll
     for (int i = 0; i < kBatch; ++i) {
       uint32_t k = 0;
+      if (lim[i] == kItems) {
 #pragma unroll
-      for (int q = 0; q < kItems; ++q) k += (keep.of(v[i][q]) && base[i] + q < n) ? 1u : 0u;
+        for (int q = 0; q < kItems; ++q) k += keep.of(v[i][q]) ? 1u : 0u;
+      } else {
+#pragma unroll
+        for (int q = 0; q < kItems; ++q) k += (q < lim[i] && keep.of(v[i][q])) ? 1u : 0u;
+      }
       const uint32_t c = (uint32_t)wave_sum((int)k);
       tot += c;
       mine = lane == i0 + i ? c : mine;
@@ -760,9 +767,13 @@ __global__ __launch_bounds__(kCountBlock) void compress_counted_kernel(
   __shared__ ST s_stage[kWaves][kChunk];
   const int lane = __lane_id();
   const int wid = threadIdx.x / kWave;
-  const int64_t ch = (int64_t)blockIdx.x * kWaves + wid;
+  // Chunks from the last one down: the count pass read its input front to
+  // back, so the caches hold its end, which this order reads first (filters
+  // of 2^28 fp16: 0.2355 -> 0.2243 ms per call; mask compactions unchanged).
+  const int64_t nch = (n + kChunk - 1) / kChunk;
+  const int64_t ch = nch - 1 - ((int64_t)blockIdx.x * kWaves + wid);
+  if (ch < 0) return;
   const int64_t c0 = ch * kChunk;
-  if (c0 >= n) return;
 
   // The chunk's output offset: its block's offset plus the chunks before it
   // in the block (loaded first: loads return in order).
