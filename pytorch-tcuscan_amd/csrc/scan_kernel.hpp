@@ -70,15 +70,22 @@ __device__ __forceinline__ uint32_t group_heads(const int8_t* __restrict__ flags
       q = (q + 1 == seg_len) ? 0 : q + 1;
     }
   } else if constexpr (SEG == kSegFlags) {
-    int8_t f[G];
+    static_assert(G % 4 == 0, "flag groups are whole 32-bit words");
+    Packed<int8_t, G> f;
     if (full) {
       load_items<int8_t, G, true>(flags, gb, n, f);
     } else {
       load_items<int8_t, G, false>(flags, gb, n, f);
     }
+    // Four flags per word: bit 7 of each byte becomes (byte != 0) (the low
+    // seven bits plus 0x7f carry into bit 7 unless zero; no carry leaves the
+    // byte), then a multiply gathers bits 7, 15, 23, 31 into bits 21..24.
 #pragma unroll
-    for (int i = 0; i < G; ++i)
-      if (f[i] != 0) h |= 1u << i;
+    for (int w = 0; w < G / 4; ++w) {
+      const uint32_t v = f.w[w];
+      const uint32_t nz = ((((v & 0x7f7f7f7fu) + 0x7f7f7f7fu) | v) & 0x80808080u) >> 7;
+      h |= ((nz * 0x00204081u) >> 21 & 0xfu) << (4 * w);
+    }
   }
   return h;
 }
