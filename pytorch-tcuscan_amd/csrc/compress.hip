@@ -540,12 +540,21 @@ inline CountedLayout counted_layout(int64_t n) {
   return l;
 }
 
-// Number of bytes equal to 1 in a 32-bit word (exact: no carries cross
-// bytes).
-__device__ __forceinline__ uint32_t ones_in_word(uint32_t v) {
+// Bit 7 of each byte of the result: that byte of v equals 1 (exact: the
+// low seven bits plus 0x7f carry into bit 7 unless zero, and no carry
+// leaves a byte).
+__device__ __forceinline__ uint32_t ones_mask(uint32_t v) {
   const uint32_t z = v ^ 0x01010101u;  // bytes equal to 1 become 0
-  const uint32_t t = ~(((z & 0x7f7f7f7fu) + 0x7f7f7f7fu) | z | 0x7f7f7f7fu);
-  return (uint32_t)__builtin_popcount(t);
+  return ~(((z & 0x7f7f7f7fu) + 0x7f7f7f7fu) | z | 0x7f7f7f7fu);
+}
+
+// Number of bytes equal to 1 in a 32-bit word.
+__device__ __forceinline__ uint32_t ones_in_word(uint32_t v) { return (uint32_t)__builtin_popcount(ones_mask(v)); }
+
+// Bits 0..3: bytes 0..3 of v equal 1 (a multiply gathers bits 7, 15, 23,
+// 31 of ones_mask into bits 21..24).
+__device__ __forceinline__ uint32_t ones_bits4(uint32_t v) {
+  return ((ones_mask(v) >> 7) * 0x00204081u) >> 21 & 0xfu;
 }
 
 // Block b (256 threads, 64K elements): wave w counts chunks w, w + 4, ...,
@@ -812,8 +821,9 @@ __global__ __launch_bounds__(kCountBlock) void compress_counted_kernel(
 #pragma unroll
       for (int i = 0; i < G; ++i) b |= (keep.of(xv[j][i]) && (full || gb + i < n) ? 1u : 0u) << i;
     } else {
+      // Four mask bytes per word (items past n load as 0).
 #pragma unroll
-      for (int i = 0; i < G; ++i) b |= (m[i] == 1 ? 1u : 0u) << i;  // items past n load as 0
+      for (int w = 0; w < G / 4; ++w) b |= ones_bits4(m.w[w]) << (4 * w);
     }
     mb[j] = b;
   }
@@ -829,7 +839,9 @@ __global__ __launch_bounds__(kCountBlock) void compress_counted_kernel(
   }
   ST* stage = s_stage[wid];
   const int64_t room = cap - base;
-  const int wn = room <= 0 ? 0 : (room < wtot ? (int)room : wtot);
+  // A workspace that does not hold this input's counts (no count pass just
+  // before) can give any base: nothing outside [0, cap) is written.
+  const int wn = (room <= 0 || base < 0) ? 0 : (room < wtot ? (int)room : wtot);
   auto wave_sync = [&]() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
