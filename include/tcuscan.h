@@ -450,6 +450,22 @@ int tcs_topk_pivot(const void* x, int64_t n, int num_samples, void* out, int64_t
 int tcs_searchsorted(const int32_t* sorted, int64_t sorted_len, const int32_t* values, int32_t* out,
                      int64_t num_values, void* stream);
 
+/* Workspace bytes of tcs_searchsorted_indexed for a haystack of sorted_len
+ * elements: a bucket table of 2^b + 1 int32, b = clamp(floor(log2(len)) - 4,
+ * 10, 20) (at most 4 MiB + 4 bytes). */
+size_t tcs_searchsorted_workspace_bytes(int64_t sorted_len);
+
+/* tcs_searchsorted with a bucket index built in `workspace` first (no host
+ * synchronisation): the haystack's order keys between its first and last
+ * element are cut into 2^b buckets of equal width, the start of every
+ * bucket is found once, and each needle then searches only its bucket
+ * (about 16 elements when the keys spread over their range; clustered keys
+ * fall back to the sampled search of tcs_searchsorted).  Results identical
+ * to tcs_searchsorted.  Replaces searchsorted_int32 behind run_searchsorted
+ * (src/torch/torch_searchsorted.h:33-82) for large haystacks. */
+int tcs_searchsorted_indexed(const int32_t* sorted, int64_t sorted_len, const int32_t* values, int32_t* out,
+                             int64_t num_values, void* workspace, size_t workspace_bytes, void* stream);
+
 /* ------------------------------------------------------------------------
  * Batched work plans
  *

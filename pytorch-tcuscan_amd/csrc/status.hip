@@ -39,7 +39,7 @@ const char* tcs_status_string(int status) {
   }
 }
 
-int tcs_abi_version(void) { return 103; }
+int tcs_abi_version(void) { return 104; }
 
 int tcs_workspace_reset(void* workspace, size_t workspace_bytes, void* stream) {
   if (!workspace || workspace_bytes < tcs::kHeaderBytes) return TCS_ERR_WORKSPACE;

@@ -1191,6 +1191,9 @@ at::Tensor run_count_if(const at::Tensor& x_in, float pivot, uint32_t tile_len, 
   return cnt.to(at::kInt);
 }
 
+// run_searchsorted takes the bucket index above these sizes.
+constexpr int64_t kIndexedMinLen = (int64_t)1 << 16, kIndexedMinNeedles = (int64_t)1 << 16;
+
 // src/torch/torch_searchsorted.h:33-82
 at::Tensor run_searchsorted(const at::Tensor& sorted_in, const at::Tensor& values_in) {
   const char* op = "run_searchsorted";
@@ -1202,9 +1205,19 @@ at::Tensor run_searchsorted(const at::Tensor& sorted_in, const at::Tensor& value
   at::Tensor sorted = sorted_in.contiguous(), values = values_in.contiguous();
   at::Tensor out = at::empty({values.numel()}, values.options());  // flat, as the reference returns
   const at::hip::HIPGuardMasqueradingAsCUDA guard(values.device());
-  check(tcs_searchsorted(sorted.data_ptr<int32_t>(), sorted.numel(), values.data_ptr<int32_t>(),
-                         out.data_ptr<int32_t>(), values.numel(), stream_of(values)),
-        op);
+  // Haystacks beyond the LDS samples with many needles: the bucket index
+  // (its build costs about one search per 16 haystack elements).
+  if (sorted.numel() > kIndexedMinLen && values.numel() >= kIndexedMinNeedles) {
+    auto ws = workspace(values, tcs_searchsorted_workspace_bytes(sorted.numel()));
+    check(tcs_searchsorted_indexed(sorted.data_ptr<int32_t>(), sorted.numel(), values.data_ptr<int32_t>(),
+                                   out.data_ptr<int32_t>(), values.numel(), ws.first, ws.second,
+                                   stream_of(values)),
+          op);
+  } else {
+    check(tcs_searchsorted(sorted.data_ptr<int32_t>(), sorted.numel(), values.data_ptr<int32_t>(),
+                           out.data_ptr<int32_t>(), values.numel(), stream_of(values)),
+          op);
+  }
   return out;
 }
 
