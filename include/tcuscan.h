@@ -461,7 +461,11 @@ size_t tcs_searchsorted_workspace_bytes(int64_t sorted_len);
  * bucket is found once, and each needle then searches only its bucket
  * (about 16 elements when the keys spread over their range; clustered keys
  * fall back to the sampled search of tcs_searchsorted).  Results identical
- * to tcs_searchsorted.  Replaces searchsorted_int32 behind run_searchsorted
+ * to tcs_searchsorted.  `workspace` is scratch that this call overwrites
+ * from its first byte (any contents before, nothing kept after): it must
+ * not be a look-back workspace that other calls use, unless that is zeroed
+ * again (tcs_workspace_reset) before its next look-back use.  Replaces
+ * searchsorted_int32 behind run_searchsorted
  * (src/torch/torch_searchsorted.h:33-82) for large haystacks. */
 int tcs_searchsorted_indexed(const int32_t* sorted, int64_t sorted_len, const int32_t* values, int32_t* out,
                              int64_t num_values, void* workspace, size_t workspace_bytes, void* stream);

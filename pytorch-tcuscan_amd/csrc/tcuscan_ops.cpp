@@ -1208,9 +1208,12 @@ at::Tensor run_searchsorted(const at::Tensor& sorted_in, const at::Tensor& value
   // Haystacks beyond the LDS samples with many needles: the bucket index
   // (its build costs about one search per 16 haystack elements).
   if (sorted.numel() > kIndexedMinLen && values.numel() >= kIndexedMinNeedles) {
-    auto ws = workspace(values, tcs_searchsorted_workspace_bytes(sorted.numel()));
+    // The bucket table is scratch of this call only: a buffer of its own, not
+    // the stream's look-back workspace (whose header it would overwrite).
+    const size_t tb = tcs_searchsorted_workspace_bytes(sorted.numel());
+    at::Tensor table = at::empty({(int64_t)tb}, values.options().dtype(at::kByte));
     check(tcs_searchsorted_indexed(sorted.data_ptr<int32_t>(), sorted.numel(), values.data_ptr<int32_t>(),
-                                   out.data_ptr<int32_t>(), values.numel(), ws.first, ws.second,
+                                   out.data_ptr<int32_t>(), values.numel(), table.data_ptr(), tb,
                                    stream_of(values)),
           op);
   } else {

@@ -114,3 +114,28 @@ def test_indexed_empty_and_rejects(lib, dev):
     # a workspace below tcs_searchsorted_workspace_bytes
     assert lib.tcs_searchsorted_indexed(hay.data_ptr(), 100000, v.data_ptr(), out.data_ptr(), 10, ws.data_ptr(),
                                         256, s) != 0
+
+
+def test_op_leaves_the_stream_workspace_intact(ops, dev):
+    """run_searchsorted on a large haystack builds its bucket table in a
+    buffer of its own: look-back ops on the same stream afterwards (the
+    persistent scan's ticket counters and launch epoch live in the shared
+    workspace's header) still give exact results, and so does a second
+    search."""
+    import oracle
+    from _util import gen
+
+    rng = np.random.default_rng(5)
+    hay = _haystack("uniform", 1 << 18, rng)
+    needles = _needles(rng, hay, 1 << 17)
+    exp = np.searchsorted(hay, needles, side="left").astype(np.int32)
+    h, v = torch.from_numpy(hay).to(dev), torch.from_numpy(needles).to(dev)
+    x = (torch.rand((1 << 20) + 5, generator=gen(3)) * 2 - 1).half()
+    xd = x.to(dev)
+    for _ in range(3):
+        assert np.array_equal(ops.run_searchsorted(h, v).cpu().numpy(), exp)
+        y = ops.run_scan_multi_core(xd, 128)
+        assert np.array_equal(y.cpu().numpy(), oracle.scan(x.numpy()))
+        m = (torch.rand(xd.numel(), generator=gen(4)) < 0.3).to(torch.int8).to(dev)
+        z = ops.run_compress(xd, m, 128)
+        assert torch.equal(z.cpu(), x[m.cpu() == 1])
