@@ -65,30 +65,41 @@ __device__ __forceinline__ int64_t row_end(const IdxT* __restrict__ ends, int64_
 // marks a tile past the last row.
 // With `part` set (the split SpMV) empty rows zero both partial sums
 // part[r] and part[R + r] instead of y[r].
+// Each thread takes kRowsPerThread consecutive rows, their ends loaded
+// together (4M rows: 9.8 us, 10.8 with one row per thread).
+constexpr int kRowsPerThread = 4;
+
 template <int64_t TILE, class IdxT, class OutT>
 __global__ __launch_bounds__(256) void seg_rows_kernel(const IdxT* __restrict__ ends, int64_t R,
                                                        int64_t n_ends, int64_t nnz,
                                                        OutT* __restrict__ y, double* __restrict__ part,
                                                        uint64_t* __restrict__ tile_row,
                                                        uint32_t num_tiles) {
-  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= R) return;
-  const int64_t s = row_end(ends, r - 1, n_ends, nnz);
-  const int64_t e = row_end(ends, r, n_ends, nnz);
-  if (e <= s) {
-    if (part) {
-      part[r] = 0.0;
-      part[R + r] = 0.0;
+  const int64_t r0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * kRowsPerThread;
+  if (r0 >= R) return;
+  int64_t e[kRowsPerThread + 1];  // e[k] = end of row r0 - 1 + k
+#pragma unroll
+  for (int k = 0; k <= kRowsPerThread; ++k) e[k] = r0 - 1 + k < R ? row_end(ends, r0 - 1 + k, n_ends, nnz) : 0;
+#pragma unroll
+  for (int k = 0; k < kRowsPerThread; ++k) {
+    const int64_t r = r0 + k;
+    if (r >= R) break;
+    const int64_t s = e[k], t = e[k + 1];
+    if (t <= s) {
+      if (part) {
+        part[r] = 0.0;
+        part[R + r] = 0.0;
+      } else {
+        y[r] = OutT(0);
+      }
     } else {
-      y[r] = OutT(0);
+      for (int64_t b = (s + TILE - 1) / TILE; b * TILE < t && b < num_tiles; ++b)
+        tile_row[b] = (uint64_t)r;
     }
-  } else {
-    for (int64_t b = (s + TILE - 1) / TILE; b * TILE < e && b < num_tiles; ++b)
-      tile_row[b] = (uint64_t)r;
-  }
-  if (r == R - 1) {
-    const int64_t last = e > s ? e : s;
-    for (int64_t b = (last + TILE - 1) / TILE; b < num_tiles; ++b) tile_row[b] = (uint64_t)R;
+    if (r == R - 1) {
+      const int64_t last = t > s ? t : s;
+      for (int64_t b = (last + TILE - 1) / TILE; b < num_tiles; ++b) tile_row[b] = (uint64_t)R;
+    }
   }
 }
 
@@ -627,7 +638,8 @@ static int launch_seg(const Src& src, const void* ends_v, int64_t R, int64_t n_e
   auto* tile_row = reinterpret_cast<uint64_t*>(static_cast<char*>(ws) + plan.tile_row_off);
   double* part = plan.parts > 1 ? reinterpret_cast<double*>(static_cast<char*>(ws) + plan.part_off) : nullptr;
   const int64_t row_blocks = (R + 255) / 256;
-  hipLaunchKernelGGL((seg_rows_kernel<Src::kTile, IdxT, OutT>), dim3((uint32_t)row_blocks), dim3(256), 0, st,
+  const int64_t rows_blocks = (R + 256 * kRowsPerThread - 1) / (256 * kRowsPerThread);
+  hipLaunchKernelGGL((seg_rows_kernel<Src::kTile, IdxT, OutT>), dim3((uint32_t)rows_blocks), dim3(256), 0, st,
                      ends, R, n_ends, nnz, y, part, tile_row, plan.num_tiles);
   TCS_CHECK_LAUNCH();
   if (plan.num_tiles == 0) return TCS_OK;
