@@ -264,6 +264,27 @@ def _filter_workload(n: int, S: int, steps: int, warmup: int, pivot: float = 0.5
             "gbs_wall": bytes_per_step * steps / m["wall_s"] / 1e9, "kept": k}
 
 
+def _split_workload(n: int, S: int, steps: int, warmup: int, p: float = 0.3):
+    """run_split of N fp16 standard normals by an int8 mask with P(mask == 1)
+    = 0.3: a count pass (N mask bytes read) and a placing pass (N mask bytes
+    and 2N bytes of x read, 2N written).  Bytes per step: 6N."""
+    import torch
+
+    import tcuscan
+
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device="cpu").manual_seed(0)
+    x = torch.randn(n, generator=g).half().to(dev)
+    m = (torch.rand(n, generator=g) < p).to(torch.int8).to(dev)
+    torch.cuda.synchronize()
+    r = _measure(lambda: tcuscan.run_split(x, m, S), steps, warmup, flush=False)
+    bytes_per_step = 6 * n
+    del x, m
+    torch.cuda.empty_cache()
+    return {"bytes_per_step": bytes_per_step, "wall_s": r["wall_s"], "event_avg_ms": r["hot"]["mean_ms"],
+            "gbs_wall": bytes_per_step * steps / r["wall_s"] / 1e9}
+
+
 def _searchsorted_workload(len_log2: int, needles_log2: int, steps: int, warmup: int):
     """run_searchsorted of 2^needles_log2 random int32 needles into 2^len_log2
     sorted random int32 values; reported as needles per second."""
@@ -802,7 +823,8 @@ def main() -> int:
     ap.add_argument("--only", choices=["scan_f16", "scan_i8", "scan_bf16", "scan_f16_mfma", "scan_i8_mfma",
                                        "scan_f16_dpp", "scan_i8_dpp", "compress_ind", "spmv", "copy_f32",
                                        "batched", "serve", "filter_ge", "searchsorted", "sort_f16", "sort_i32",
-                                       "topk_f16", "top_p", "stream_ceilings", "segsum_i8", "segsum_f16"],
+                                       "topk_f16", "top_p", "stream_ceilings", "segsum_i8", "segsum_f16",
+                                       "split_f16"],
                     help="run one workload and print its timing only (for profilers)")
     a = ap.parse_args()
     if a.gpus != 1:
@@ -834,6 +856,8 @@ def main() -> int:
             r = _filter_workload(n, a.S, a.steps, a.warmup)
         elif a.only == "searchsorted":
             r = _searchsorted_workload(24, 24, a.steps, a.warmup)
+        elif a.only == "split_f16":
+            r = _split_workload(n, a.S, a.steps, a.warmup)
         elif a.only == "topk_f16":
             r = _topk_workload(n, 2048, a.steps, a.warmup)
         elif a.only == "top_p":
@@ -887,6 +911,14 @@ def main() -> int:
             "bytes_per_step": fl["bytes_per_step"], "kept": fl["kept"],
             "ms_per_step": round(fl["wall_s"] * 1e3 / ksteps, 4),
             "note": "count pass (chunk counts) + counted compare-and-compact pass, with the count read back to size the output",
+        }
+        _progress("split")
+        sp = _split_workload(n, a.S, ksteps, max(3, a.warmup // 2))
+        extras[f"split_f16_2p{a.log2n}_p0.3"] = {
+            "value": round(sp["gbs_wall"], 2), "unit": "GB/s",
+            "bytes_per_step": sp["bytes_per_step"],
+            "ms_per_step": round(sp["wall_s"] * 1e3 / ksteps, 4),
+            "note": "count pass (chunk counts, K) + counted placing pass; 6 B per element (mask read twice)",
         }
         _progress("radix sort")
         so = _sort_workload(n, "fp16", max(5, ksteps // 2), 2)

@@ -4,83 +4,65 @@
 // tests/test_split.py:52-66 compare the two parts with masked_select of
 // mask == 1 and mask == 0).
 //
-// Two launches, no host sync: a count pass writes K = #(mask == 1) to the
-// workspace, then one single-pass kernel places every element.  A tile's
-// kept elements start at the look-back prefix C of kept counts over the
-// tiles before it; its other elements start at K + (t0 - C), t0 being the
-// tile's first index.  Each wave stages its kept run and its other run in
-// LDS and writes both with coalesced stores.
-#include "tcs_common.hpp"
-#include "tcs_device.hpp"
+// Two launches, no host sync: the count pass (tcs_count_mask) writes
+// K = #(mask == 1) and the kept count of every 2048-element chunk to the
+// workspace, then every wave of one kernel places one chunk on its own:
+// the chunk's kept elements start at kb, the kept count before the chunk
+// (its block's offset plus the chunk counts before it), its other elements
+// at K + (c0 - kb), c0 being the chunk's first index.  The wave stages its
+// kept run and then its other run in LDS and writes each with line-aligned
+// 16-byte stores.
+#include "tcs_counted.hpp"
 
 namespace tcs {
 
-template <int BYTES> struct SplitBits;
-template <> struct SplitBits<1> { using T = uint8_t; };
-template <> struct SplitBits<2> { using T = uint16_t; };
-template <> struct SplitBits<4> { using T = uint32_t; };
-template <> struct SplitBits<8> { using T = uint64_t; };
-
-constexpr int kSplitNW = 8, kSplitG = 8;
-
-// Groups per wave: about 24-32 KiB of input per tile.
-template <int XB, bool IDX>
-struct SplitGeom {
-  static constexpr int kInBytes = XB + 1 + (IDX ? 4 : 0);
-  static constexpr int J = kInBytes <= 3 ? 4 : 2;
-  static constexpr int64_t kTile = (int64_t)kSplitNW * J * kWave * kSplitG;
-};
-constexpr int64_t kSplitTileMin = (int64_t)kSplitNW * 2 * kWave * kSplitG;  // 8192
-
 // IDX: also permute idx_in (int32) the same way.
 template <int XB, bool IDX>
-__global__ __launch_bounds__(kSplitNW * kWave) void split_kernel(
-    const typename SplitBits<XB>::T* __restrict__ x, const int8_t* __restrict__ mask,
-    const int32_t* __restrict__ idx_in, typename SplitBits<XB>::T* __restrict__ out_x,
-    int32_t* __restrict__ out_idx, const int64_t* __restrict__ kept_total, int64_t n, void* ws,
-    uint32_t num_tiles) {
-  using XT = typename SplitBits<XB>::T;
-  using Geom = SplitGeom<XB, IDX>;
-  constexpr int G = kSplitG, J = Geom::J, NW = kSplitNW;
-  constexpr int64_t TILE = Geom::kTile;
-  constexpr int64_t GSPAN = (int64_t)kWave * G, WSEG = GSPAN * J;
-  __shared__ int s_wval[NW], s_wexcl[NW];
-  __shared__ int64_t s_base;
-  __shared__ LaunchTag s_tag;
-  __shared__ XT s_stage[NW][WSEG];           // kept run, then the other run
-  __shared__ int32_t s_istage[IDX ? NW : 1][IDX ? WSEG : 1];
-
-  if (threadIdx.x == 0) s_tag = launch_arrive(ws);
+__global__ __launch_bounds__(kCountBlock) void split_counted_kernel(
+    const typename Bits<XB>::T* __restrict__ x, const int8_t* __restrict__ mask,
+    const int32_t* __restrict__ idx_in, typename Bits<XB>::T* __restrict__ out_x,
+    int32_t* __restrict__ out_idx, const int64_t* __restrict__ kept_total, int64_t n,
+    const int64_t* __restrict__ block_off, const uint64_t* __restrict__ chunk_cnt) {
+  using XT = typename Bits<XB>::T;
+  constexpr int G = 8, J = (int)(kChunk / (kWave * G));  // 4 groups of 8 per lane
+  constexpr int kWaves = kCountBlock / kWave;
+  using ST = typename Bits<(IDX && XB < 4) ? 4 : XB>::T;
+  __shared__ ST s_stage[kWaves][kChunk];  // kept run, then the other run
   const int lane = __lane_id();
   const int wid = threadIdx.x / kWave;
-  const uint32_t tile = blockIdx.x;
-  const int64_t t0 = (int64_t)tile * TILE;
-  const int64_t w0 = t0 + (int64_t)wid * WSEG;        // the wave's first element
-  const int64_t wbase = w0 + (int64_t)lane * G;
-  const bool full = t0 + TILE <= n;
+  // Chunks from the last one down: the count pass read the mask front to
+  // back, so the caches hold its end, which this order reads first.
+  const int64_t nch = (n + kChunk - 1) / kChunk;
+  const int64_t ch = nch - 1 - ((int64_t)blockIdx.x * kWaves + wid);
+  if (ch < 0) return;
+  const int64_t c0 = ch * kChunk;
+  const int64_t part = chunk_base_part(ch, block_off, chunk_cnt, lane);
+  const int64_t ktot = kept_total[0];
 
+  const bool full = c0 + kChunk <= n;
   uint32_t mb[J];
   Packed<XT, G> xv[J];
   Packed<int32_t, G> iv[IDX ? J : 1];
 #pragma unroll
   for (int j = 0; j < J; ++j) {
-    const int64_t gb = wbase + j * GSPAN;
+    const int64_t gb = c0 + j * (kWave * G) + lane * G;
     Packed<int8_t, G> m;
     if (full) {
+      // The mask was read by the count pass just before: plain loads, which
+      // find part of it in the caches; values and indices only this pass
+      // reads.
       load_items<int8_t, G, true>(mask, gb, n, m);
-      load_items<XT, G, true>(x, gb, n, xv[j]);
-      if constexpr (IDX) load_items<int32_t, G, true>(idx_in, gb, n, iv[j]);
+      load_nt(x, gb, xv[j]);
+      if constexpr (IDX) load_nt(idx_in, gb, iv[j]);
     } else {
       load_items<int8_t, G, false>(mask, gb, n, m);
       load_items<XT, G, false>(x, gb, n, xv[j]);
       if constexpr (IDX) load_items<int32_t, G, false>(idx_in, gb, n, iv[j]);
     }
-    uint32_t b = 0;
-#pragma unroll
-    for (int i = 0; i < G; ++i) b |= (m[i] == 1 ? 1u : 0u) << i;  // items past n load as 0
-    mb[j] = b;
+    // Four mask bytes per word (items past n load as 0: not kept).
+    mb[j] = ones_bits4(m.w[0]) | ones_bits4(m.w[1]) << 4;
   }
-
+  const int64_t kb = wave_sum(part);
   int gex[J];
   int wtot = 0;
 #pragma unroll
@@ -90,77 +72,46 @@ __global__ __launch_bounds__(kSplitNW * kWave) void split_kernel(
     gex[j] = wtot + incl - c;
     wtot += readlane(incl, kWave - 1);
   }
-  if (lane == 0) s_wval[wid] = wtot;
-  __syncthreads();
-
-  if (wid == 0) {
-    int a = lane < NW ? s_wval[lane] : 0;
-    const int incl = wave_incl_scan(a);
-    const int agg = readlane(incl, NW - 1);
-    if (lane < NW) s_wexcl[lane] = incl - a;
-    auto fb = [&](int64_t u) {
-      const int64_t u0 = u * TILE;
-      const int64_t u1 = u0 + TILE < n ? u0 + TILE : n;
-      int c = 0;
-      for (int64_t p = u0 + lane; p < u1; p += kWave) c += mask[p] == 1 ? 1 : 0;
-      return FlagVal<int>{u == 0 ? 1 : 0, wave_sum(c)};
-    };
-    uint64_t* desc = ws_desc(ws);
-    const uint32_t tag = s_tag.tag;
-    int carry = 0;
-    if (tile == 0) {
-      if (lane == 0 && num_tiles > 1) desc_publish<int>(desc, 0, agg, tag, kFlagIncl);
-    } else {
-      if (lane == 0) desc_publish<int>(desc, tile, agg, tag, kFlagAgg);
-      carry = lookback<int>(desc, tile, tag, s_tag.polls, fb);
-      if (lane == 0) desc_publish<int>(desc, tile, carry + agg, tag, kFlagIncl);
-    }
-    if (lane == 0) s_base = carry;
-  }
-  __syncthreads();
-
-  // Kept: [kbase, kbase + wtot).  Others: [rbase, rbase + wvalid - wtot).
-  const int64_t kbase = s_base + s_wexcl[wid];
-  const int64_t rbase = kept_total[0] + (w0 - kbase);
-  const int64_t wend = w0 + WSEG < n ? w0 + WSEG : n;
-  const int wvalid = wend > w0 ? (int)(wend - w0) : 0;
-  XT* stage = s_stage[wid];
+  const int valid = full ? (int)kChunk : (int)(n - c0);
+  const int64_t rb = ktot + (c0 - kb);
+  ST* stage = s_stage[wid];
+  // Kept element e of the chunk goes to stage[kept before e], any other to
+  // stage[wtot + (e - kept before e)]; then both runs go out.
+  auto place = [&](auto out, auto value_of) {
 #pragma unroll
-  for (int j = 0; j < J; ++j) {
-    int kpos = gex[j];
-    const int e0 = j * (int)GSPAN + lane * G;  // wave-relative index of item 0
+    for (int j = 0; j < J; ++j) {
+      int kpos = gex[j];
+      const int e0 = j * kWave * G + lane * G;
 #pragma unroll
-    for (int i = 0; i < G; ++i) {
-      const bool kept = (mb[j] >> i) & 1u;
-      const int e = e0 + i;
-      // Others before e in the wave = e - kept before e.
-      const int pos = kept ? kpos : wtot + (e - kpos);
-      if (e < wvalid) {
-        stage[pos] = xv[j][i];
-        if constexpr (IDX) s_istage[wid][pos] = iv[j][i];
+      for (int i = 0; i < G; ++i) {
+        const bool kept = (mb[j] >> i) & 1u;
+        const int e = e0 + i;
+        const int pos = kept ? kpos : wtot + (e - kpos);
+        if (full || e < valid) stage[pos] = (ST)value_of(j, i);
+        kpos += kept ? 1 : 0;
       }
-      kpos += kept ? 1 : 0;
     }
-  }
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  for (int k = lane; k < wvalid; k += kWave) {
-    const int64_t dst = k < wtot ? kbase + k : rbase + (k - wtot);
-    __builtin_nontemporal_store(stage[k], out_x + dst);
-    if constexpr (IDX) __builtin_nontemporal_store(s_istage[wid][k], out_idx + dst);
-  }
+    wave_lds_sync();
+    flush_run(stage, out, kb, wtot, lane);
+    flush_run(stage + wtot, out, rb, valid - wtot, lane);
+    wave_lds_sync();
+  };
+  place(out_x, [&](int j, int i) { return xv[j][i]; });
+  if constexpr (IDX) place(out_idx, [&](int j, int i) { return (uint32_t)iv[j][i]; });
 }
 
 template <int XB, bool IDX>
 static int launch_split(const void* x, const int8_t* mask, const int32_t* idx_in, void* out_x, int32_t* out_idx,
                         const int64_t* kept, int64_t n, void* ws, hipStream_t st) {
-  constexpr int64_t kTile = SplitGeom<XB, IDX>::kTile;
-  const uint32_t tiles = (uint32_t)((n + kTile - 1) / kTile);
-  using XT = typename SplitBits<XB>::T;
-  hipLaunchKernelGGL((split_kernel<XB, IDX>), dim3(tiles), dim3(kSplitNW * kWave), 0, st,
-                     static_cast<const XT*>(x), mask, idx_in, static_cast<XT*>(out_x), out_idx, kept, n, ws,
-                     tiles);
+  using XT = typename Bits<XB>::T;
+  const CountedLayout l = counted_layout(n);
+  const int64_t nch = (n + kChunk - 1) / kChunk;
+  const int64_t blocks = (nch + kCountBlock / kWave - 1) / (kCountBlock / kWave);
+  const char* w = static_cast<const char*>(ws);
+  hipLaunchKernelGGL((split_counted_kernel<XB, IDX>), dim3((uint32_t)blocks), dim3(kCountBlock), 0, st,
+                     static_cast<const XT*>(x), mask, idx_in, static_cast<XT*>(out_x), out_idx, kept, n,
+                     reinterpret_cast<const int64_t*>(w + l.block_off),
+                     reinterpret_cast<const uint64_t*>(w + l.chunk_cnt));
   TCS_CHECK_LAUNCH();
   return TCS_OK;
 }
@@ -170,8 +121,8 @@ static int launch_split(const void* x, const int8_t* mask, const int32_t* idx_in
 extern "C" {
 
 size_t tcs_split_workspace_bytes(int64_t n) {
-  // The count pass's partials and the split's descriptors share the space
-  // after the header (they are used one after the other); K sits after it.
+  // The count pass's block counts, block offsets and chunk counts after the
+  // header (the counted layout); K sits after them.
   return ((tcs_compress_workspace_bytes(n) + 255) & ~(size_t)255) + 256;
 }
 
