@@ -11,8 +11,11 @@
 // key above T and the first `need` keys equal to T in input order (per-tile
 // counts, their scan, and placement by the tiles that hold any of them),
 // and the stable radix sort orders those k, largest first (ties by input
-// position).  Reads: one pass per key byte plus one, plus the tiles that
-// hold selected keys.
+// position).  The per-tile counts of keys of two or more bytes come from a
+// record the last digit round keeps (keys above the higher digits, low
+// bytes of the few matching them), so reads are one pass per key byte plus
+// the tiles that hold selected keys (one-byte keys: one more pass for the
+// tile counts).
 //
 // tcs_topk_pivot follows run_topk_pivot_fp16 (src/torch/torch_topk.h:
 // 23-60, src/kernels/kernel_topk_pivot.h): per block, over its first eight
@@ -27,9 +30,6 @@ namespace tcs {
 
 constexpr int kTkThreads = 512, kTkMaxGroups = 1024, kTkPickThreads = 1024;
 constexpr int kTkNW = 8, kTkG = 8;
-// Placement walks the tiles with a fixed grid: most tiles place nothing
-// when k is small, and a workgroup per tile would cost a launch each.
-constexpr uint32_t kTkPlaceGrid = 2048;
 
 template <int DT>
 struct TopkGeom {
@@ -39,34 +39,70 @@ struct TopkGeom {
 };
 constexpr int64_t kTkTileMin = (int64_t)kTkNW * 2 * kWave * kTkG;  // 8192
 
+// The last digit round of a key of two or more bytes also records, per tile
+// (one unrolled step of a histogram workgroup covers one tile), the number
+// of keys whose higher digits are above the chosen ones and the low bytes of
+// up to kTkListMax keys whose higher digits equal them.  Once the last digit
+// is chosen, each tile's counts above and at the threshold follow from that
+// record without reading the keys again; a tile with more matching keys than
+// were recorded is counted from its keys (topk_tilefix_kernel).
+constexpr int kTkListMax = 16;
+// Placement reads only the sub-chunks of a tile that hold selected keys
+// (their record's mask bits): a selected key is above or matches the
+// higher digits.
+constexpr int64_t kTkSubChunk = 2048;
+// Tiles per histogram chunk: chunks are whole tiles, at most
+// ceil(n / kTkMaxGroups) <= 2^21 keys for n < 2^31, so 256 of 8192.
+constexpr int kTkMaxChunkTiles = 256;
+
+inline int64_t topk_tile(int key_bytes) { return (int64_t)kTkNW * (key_bytes <= 2 ? 4 : 2) * kWave * kTkG; }
+
 // ---------------------------------------------------------------------------
 // Digit histogram among the keys matching the chosen higher digits.
 // state[0]: key digits chosen so far; state[1]: how many of the keys
 // matching them are still to be taken.  64-bit words, high half zero.
 // ---------------------------------------------------------------------------
-template <int DT>
+// FUSE (the last round, r = 0, of keys of two or more bytes): also record
+// per tile the keys above the chosen higher digits, which of its
+// kTkSubChunk-key sub-chunks hold any key above or matching them, and the
+// low bytes of up to kTkListMax keys matching them: tstat[tile] = above <<
+// 32 | sub-chunk mask << 16 | matching, tlist[tile * kTkListMax + j] = low
+// byte (any order).
+template <int DT, bool FUSE>
 __global__ __launch_bounds__(kTkThreads) void topk_hist_kernel(const typename SortKey<DT>::Raw* __restrict__ x,
                                                                int64_t n, int64_t chunk, uint32_t flip, int r,
                                                                const uint64_t* __restrict__ state,
-                                                               uint64_t* __restrict__ partial) {
+                                                               uint64_t* __restrict__ partial,
+                                                               uint64_t* __restrict__ tstat,
+                                                               uint8_t* __restrict__ tlist) {
   using KT = SortKey<DT>;
   using Raw = typename KT::Raw;
   constexpr int NWV = kTkThreads / kWave, C = 4, NH = NWV * C;
   constexpr int V = 16 / (int)sizeof(Raw);
+  // Four 16-byte loads in flight per thread per step; steps that lie wholly
+  // inside the chunk skip the per-element bounds checks.
+  constexpr int U = 4;
+  constexpr int64_t STEP = (int64_t)kTkThreads * V;
+  constexpr int64_t TILE = TopkGeom<DT>::kTile;
+  static_assert(!FUSE || (U * STEP == TILE && KT::kBits > 8), "one whole step per tile");
+  static_assert(TILE < 65536, "packed 16-bit tile counts");
+  static_assert(TILE / kTkSubChunk <= 16, "sub-chunk mask in 16 bits");
+  constexpr int NT = FUSE ? kTkMaxChunkTiles : 1;
   // 257 words per copy: the copies of one bin fall in different LDS banks.
   __shared__ uint32_t h[NH][257];
+  __shared__ uint32_t s_am[NT];                         // per tile: above << 16 | matching
+  __shared__ uint32_t s_sub[NT];                        // per tile: sub-chunks holding either
+  __shared__ uint8_t s_list[NT][FUSE ? kTkListMax : 1];  // low bytes of matching keys
   const int tid = threadIdx.x, wid = tid / kWave;
   const int hc = wid * C + (__lane_id() % C);
   for (int k = tid; k < NH * 257; k += kTkThreads) (&h[0][0])[k] = 0;
+  if constexpr (FUSE)
+    for (int k = tid; k < NT; k += kTkThreads) s_am[k] = s_sub[k] = 0;
   __syncthreads();
   const int hs = 8 * (r + 1);  // bits above digit r
   const uint32_t want = hs >= KT::kBits ? 0u : (uint32_t)(state[0] >> hs);
   const int64_t c0 = (int64_t)blockIdx.x * chunk;
   const int64_t c1 = c0 + chunk < n ? c0 + chunk : n;
-  // Four 16-byte loads in flight per thread per step; steps that lie wholly
-  // inside the chunk skip the per-element bounds checks.
-  constexpr int U = 4;
-  constexpr int64_t STEP = (int64_t)kTkThreads * V;
   auto count = [&](const Raw (&v)[V]) {
 #pragma unroll
     for (int i = 0; i < V; ++i) {
@@ -74,23 +110,71 @@ __global__ __launch_bounds__(kTkThreads) void topk_hist_kernel(const typename So
       if (hs >= KT::kBits || (k >> hs) == want) atomicAdd(&h[hc][(k >> (8 * r)) & 0xffu], 1u);
     }
   };
+  // FUSE: histogram one step's keys (Q groups of V, valid: in range) and add
+  // them to the record of local tile lt; off: the tile offset of group 0's
+  // first key (group u starts u * STEP later, inside one sub-chunk).  The
+  // low bytes are taken group by group, so only one group's keys are live
+  // at a time.
+  auto count_record = [&](auto& v, int lt, int64_t off, auto valid) {
+    constexpr int Q = (int)(sizeof(v) / sizeof(v[0]));
+    uint32_t above = 0, subm = 0;
+#pragma unroll
+    for (int u = 0; u < Q; ++u) {
+      uint32_t eqm = 0, abv = 0;
+#pragma unroll
+      for (int i = 0; i < V; ++i) {
+        const uint32_t k = KT::key(v[u][i]) ^ flip;
+        const bool ok = valid(u, i);
+        const bool eq = ok && (k >> 8) == want;
+        if (eq) atomicAdd(&h[hc][k & 0xffu], 1u);
+        abv += ok && (k >> 8) > want ? 1u : 0u;
+        eqm |= (eq ? 1u : 0u) << i;
+      }
+      above += abv;
+      if ((eqm | abv) != 0) subm |= 1u << (int)((off + u * STEP) / kTkSubChunk);
+      if (eqm != 0) {
+        uint32_t slot = atomicAdd(&s_am[lt], (uint32_t)__builtin_popcount(eqm)) & 0xffffu;
+        if (slot < (uint32_t)kTkListMax) {
+#pragma unroll
+          for (int i = 0; i < V; ++i) {
+            if ((eqm >> i) & 1u) {
+              if (slot < (uint32_t)kTkListMax) s_list[lt][slot] = (uint8_t)((KT::key(v[u][i]) ^ flip) & 0xffu);
+              ++slot;
+            }
+          }
+        }
+      }
+    }
+    if (above != 0) atomicAdd(&s_am[lt], above << 16);
+    if (subm != 0) atomicOr(&s_sub[lt], subm);
+  };
   int64_t b0 = c0 + (int64_t)tid * V;
   const int64_t full_end = c0 + ((c1 - c0) / (U * STEP)) * (U * STEP);  // steps [c0, full_end) are whole
   for (; b0 < full_end; b0 += U * STEP) {
     Raw v[U][V];
 #pragma unroll
     for (int u = 0; u < U; ++u) load_items<Raw, V, true>(x, b0 + u * STEP, c1, v[u]);
+    if constexpr (FUSE) {
+      // Chunks are whole tiles and one step is one tile.
+      count_record(v, (int)((b0 - c0) / TILE), (b0 - c0) % TILE, [](int, int) { return true; });
+    } else {
 #pragma unroll
-    for (int u = 0; u < U; ++u) count(v[u]);
+      for (int u = 0; u < U; ++u) count(v[u]);
+    }
   }
   for (; b0 < c1; b0 += STEP) {
-    Raw v[V];
-    load_items<Raw, V, false>(x, b0, c1, v);
+    Raw v[1][V];
+    load_items<Raw, V, false>(x, b0, c1, v[0]);
+    if constexpr (FUSE) {
+      // [full_end, c1) is less than one tile: the chunk's last.
+      count_record(v, (int)((full_end - c0) / TILE), b0 - full_end, [&](int, int i) { return b0 + i < c1; });
+    } else {
 #pragma unroll
-    for (int i = 0; i < V; ++i) {
-      const uint32_t k = KT::key(v[i]) ^ flip;
-      const bool match = b0 + i < c1 && (hs >= KT::kBits || (k >> hs) == want);
-      if (match) atomicAdd(&h[hc][(k >> (8 * r)) & 0xffu], 1u);
+      for (int i = 0; i < V; ++i) {
+        const uint32_t k = KT::key(v[0][i]) ^ flip;
+        const bool match = b0 + i < c1 && (hs >= KT::kBits || (k >> hs) == want);
+        if (match) atomicAdd(&h[hc][(k >> (8 * r)) & 0xffu], 1u);
+      }
     }
   }
   __syncthreads();
@@ -99,6 +183,16 @@ __global__ __launch_bounds__(kTkThreads) void topk_hist_kernel(const typename So
 #pragma unroll
     for (int w = 0; w < NH; ++w) s += h[w][e];
     partial[(size_t)blockIdx.x * 256 + e] = s;
+  }
+  if constexpr (FUSE) {
+    const int64_t t0 = c0 / TILE;
+    const int ntl = c1 > c0 ? (int)((c1 - c0 + TILE - 1) / TILE) : 0;
+    for (int e = tid; e < ntl; e += kTkThreads) {
+      const uint32_t am = s_am[e];
+      tstat[t0 + e] = ((uint64_t)(am >> 16) << 32) | (s_sub[e] << 16) | (am & 0xffffu);
+    }
+    for (int e = tid; e < ntl * kTkListMax; e += kTkThreads)
+      tlist[t0 * kTkListMax + e] = s_list[e / kTkListMax][e % kTkListMax];
   }
 }
 
@@ -249,28 +343,85 @@ __global__ __launch_bounds__(kTkNW * kWave) void topk_tilecount_kernel(
   }
 }
 
+// tcnt[tile] as topk_tilecount_kernel writes it, from the record of the
+// last digit round: keys above T are those whose higher digits are above
+// T's plus the recorded matching keys whose low byte is above T's, keys
+// equal to T the recorded ones with T's low byte.  A tile with more matching
+// keys than were recorded is counted from its keys.  One wave per tile.
+template <int DT>
+__global__ __launch_bounds__(256) void topk_tilefix_kernel(const typename SortKey<DT>::Raw* __restrict__ x,
+                                                           int64_t n, uint32_t flip,
+                                                           const uint64_t* __restrict__ state,
+                                                           const uint64_t* __restrict__ tstat,
+                                                           const uint8_t* __restrict__ tlist,
+                                                           uint64_t* __restrict__ tcnt, uint32_t tiles) {
+  using KT = SortKey<DT>;
+  using Raw = typename KT::Raw;
+  constexpr int64_t TILE = TopkGeom<DT>::kTile;
+  constexpr int V = 16 / (int)sizeof(Raw);
+  const int lane = __lane_id();
+  const uint32_t tile = blockIdx.x * (256 / kWave) + threadIdx.x / kWave;
+  if (tile >= tiles) return;
+  const uint32_t T = (uint32_t)state[0];
+  const uint64_t rec = tstat[tile];
+  const uint32_t m = (uint32_t)rec & 0xffffu;
+  uint32_t g, e;
+  if (m <= (uint32_t)kTkListMax) {
+    const bool in = lane < (int)m;
+    const uint32_t low = in ? (uint32_t)tlist[(size_t)tile * kTkListMax + lane] : 0u;
+    g = (uint32_t)(rec >> 32) + (uint32_t)__popcll(__ballot(in && low > (T & 0xffu)));
+    e = (uint32_t)__popcll(__ballot(in && low == (T & 0xffu)));
+  } else {
+    const int64_t t0 = (int64_t)tile * TILE;
+    const int64_t t1 = t0 + TILE < n ? t0 + TILE : n;
+    uint32_t gl = 0, el = 0;
+    for (int64_t b = t0 + (int64_t)lane * V; b < t1; b += (int64_t)kWave * V) {
+      Raw v[V];
+      load_items<Raw, V>(x, b, t1, v);
+#pragma unroll
+      for (int i = 0; i < V; ++i) {
+        const uint32_t kk = KT::key(v[i]) ^ flip;
+        const bool ok = b + i < t1;
+        gl += ok && kk > T ? 1u : 0u;
+        el += ok && kk == T ? 1u : 0u;
+      }
+    }
+    g = (uint32_t)wave_sum((int)gl);
+    e = (uint32_t)wave_sum((int)el);
+  }
+  if (lane == 0) tcnt[tile] = ((uint64_t)g << 32) | e;
+}
+
 // In place: tcnt[t] = sum of tcnt over the tiles before t, for t in
-// [0, tiles]; one workgroup, 16 consecutive tiles per thread per round.
-// The packed halves sum without a carry (each total is below 2^31).
+// [0, tiles]; one workgroup, 16384 tiles per round.  Wave w takes the
+// round's tiles [w * 1024, (w + 1) * 1024), lane l tiles w * 1024 + 64q + l
+// (coalesced loads and stores: 16 consecutive tiles per thread put every
+// lane of a load in a different line, 17.8 us per scan of 16384 tiles),
+// scanned as 16 wave scans in order.  The packed halves sum without a carry
+// (each total is below 2^31).
 constexpr int kTkScanThreads = 1024, kTkScanPer = 16;
 __global__ __launch_bounds__(kTkScanThreads) void topk_tilescan_kernel(uint64_t* __restrict__ tcnt,
                                                                        uint32_t tiles) {
   constexpr int NWV = kTkScanThreads / kWave;
+  constexpr int64_t WSPAN = (int64_t)kWave * kTkScanPer;
   constexpr int64_t ROUND = (int64_t)kTkScanThreads * kTkScanPer;
   __shared__ int64_t s_w[NWV];
   const int tid = threadIdx.x, lane = __lane_id(), wid = tid / kWave;
   int64_t carry = 0;
   for (int64_t r0 = 0; r0 < tiles; r0 += ROUND) {
-    const int64_t t0 = r0 + (int64_t)tid * kTkScanPer;
+    const int64_t wb = r0 + (int64_t)wid * WSPAN + lane;
     int64_t v[kTkScanPer];
-    int64_t sum = 0;
+#pragma unroll
+    for (int q = 0; q < kTkScanPer; ++q) v[q] = wb + q * kWave < tiles ? (int64_t)tcnt[wb + q * kWave] : 0;
+    int64_t run = 0;  // the wave's tiles before row q
 #pragma unroll
     for (int q = 0; q < kTkScanPer; ++q) {
-      v[q] = t0 + q < tiles ? (int64_t)tcnt[t0 + q] : 0;
-      sum += v[q];
+      const int64_t incl = wave_incl_scan(v[q]);
+      const int64_t row = readlane(incl, kWave - 1);
+      v[q] = run + incl - v[q];
+      run += row;
     }
-    const int64_t incl = wave_incl_scan(sum);
-    if (lane == kWave - 1) s_w[wid] = incl;
+    if (lane == 0) s_w[wid] = run;
     __syncthreads();
     int64_t before = carry, total = carry;
 #pragma unroll
@@ -279,72 +430,127 @@ __global__ __launch_bounds__(kTkScanThreads) void topk_tilescan_kernel(uint64_t*
       before += w < wid ? sw : 0;
       total += sw;
     }
-    before += incl - sum;
 #pragma unroll
-    for (int q = 0; q < kTkScanPer; ++q) {
-      if (t0 + q < tiles) tcnt[t0 + q] = (uint64_t)before;
-      before += v[q];
-    }
+    for (int q = 0; q < kTkScanPer; ++q)
+      if (wb + q * kWave < tiles) tcnt[wb + q * kWave] = (uint64_t)(before + v[q]);
     carry = total;
     __syncthreads();  // s_w is rewritten by the next round
   }
   if (tid == 0) tcnt[tiles] = (uint64_t)carry;
 }
 
+// One wave per tile, no workgroup barrier: wave w of the grid takes tiles
+// w, w + W, ... (W waves in all), reads the counts of 64 of them at once
+// and ballots which place anything; for each such tile it reads the keys
+// one 2048-key sub-chunk (four 512-key groups) at a time, only the
+// sub-chunks its record marks when there is one (tstat non-null), and
+// places them, carrying the tile's running offsets through one wave scan
+// per group.  Lane l holds keys 8l..8l+7 of a group, so the wave's (group,
+// lane, item) order is input order.  2^28 fp16, k = 2048: about 1,940
+// tiles place; reading all of each took 31-39 us of the placement's 45-58
+// (scattered 32 KB reads run far below the streaming rate), reading only
+// the first 1 KB of each 4-5 us.
+constexpr int kTkPlaceThreads = 256;
 template <int DT>
-__global__ __launch_bounds__(kTkNW * kWave) void topk_place_kernel(
+__global__ __launch_bounds__(kTkPlaceThreads) void topk_place_kernel(
     const typename SortKey<DT>::Raw* __restrict__ x, int64_t n, uint32_t flip,
     const uint64_t* __restrict__ state, int64_t k, const uint64_t* __restrict__ tcnt,
-    typename SortKey<DT>::Raw* __restrict__ cand_v, int32_t* __restrict__ cand_i, uint32_t num_tiles) {
-  constexpr int G = kTkG, J = TopkGeom<DT>::J, NW = kTkNW;
+    const uint64_t* __restrict__ tstat, typename SortKey<DT>::Raw* __restrict__ cand_v,
+    int32_t* __restrict__ cand_i, uint32_t num_tiles) {
+  using KT = SortKey<DT>;
+  using Raw = typename KT::Raw;
+  constexpr int G = kTkG;
   constexpr int64_t TILE = TopkGeom<DT>::kTile;
-  constexpr int64_t GSPAN = (int64_t)kWave * G, WSEG = GSPAN * J;
-  __shared__ int s_wval[NW];
+  constexpr int64_t GSPAN = (int64_t)kWave * G;
+  constexpr int B = (int)(kTkSubChunk / GSPAN);  // groups per sub-chunk
+  constexpr int NS = (int)(TILE / kTkSubChunk);  // sub-chunks per tile
+  static_assert(TILE % kTkSubChunk == 0 && kTkSubChunk % GSPAN == 0, "whole sub-chunks of whole groups");
+  constexpr uint32_t kAll = (1u << NS) - 1u;
+  static_assert(GSPAN < 65536, "packed 16-bit group counts");
   const int64_t need = (int64_t)state[1];
   const int64_t c_gt = k - need;
-  const int lane = __lane_id(), wid = threadIdx.x / kWave;
-  for (uint32_t tile = blockIdx.x; tile < num_tiles; tile += gridDim.x) {
-    const uint64_t e0 = tcnt[tile], e1 = tcnt[tile + 1];
-    const int64_t gt0 = (int64_t)(e0 >> 32), eq0 = (int64_t)(e0 & 0xffffffffu);
-    const int64_t gth = (int64_t)(e1 >> 32) - gt0, eqh = (int64_t)(e1 & 0xffffffffu) - eq0;
-    if (gth == 0 && (eqh == 0 || eq0 >= need)) continue;  // the same for the whole workgroup
-
-    Packed<typename SortKey<DT>::Raw, G> xv[J];
-    uint32_t gtb[J], eqb[J];
-    topk_classify_any<DT>(x, n, flip, (uint32_t)state[0], tile, wid, lane, xv, gtb, eqb);
-    int gex[J];
-    int wtot = 0;
-#pragma unroll
-    for (int j = 0; j < J; ++j) {
-      const int c = (__builtin_popcount(gtb[j]) << 16) | __builtin_popcount(eqb[j]);
-      const int incl = wave_incl_scan(c);
-      gex[j] = wtot + incl - c;
-      wtot += readlane(incl, kWave - 1);
+  const uint32_t T = (uint32_t)state[0];
+  const int lane = __lane_id();
+  const uint32_t W = gridDim.x * (kTkPlaceThreads / kWave);
+  const uint32_t gw = blockIdx.x * (kTkPlaceThreads / kWave) + threadIdx.x / kWave;
+  if (gw >= num_tiles) return;
+  const uint32_t per = (num_tiles - gw + W - 1) / W;
+  for (uint32_t j0 = 0; j0 < per; j0 += kWave) {
+    int64_t lgt0 = 0, leq0 = 0;
+    uint32_t lsub = kAll;
+    bool places = false;
+    if (j0 + lane < per) {
+      const uint32_t t = gw + (j0 + lane) * W;
+      const uint64_t e0 = tcnt[t], e1 = tcnt[t + 1];
+      if (tstat) lsub = (uint32_t)(tstat[t] >> 16) & kAll;
+      lgt0 = (int64_t)(e0 >> 32);
+      leq0 = (int64_t)(e0 & 0xffffffffu);
+      const int64_t gth = (int64_t)(e1 >> 32) - lgt0, eqh = (int64_t)(e1 & 0xffffffffu) - leq0;
+      places = !(gth == 0 && (eqh == 0 || leq0 >= need));
     }
-    if (lane == 0) s_wval[wid] = wtot;
-    __syncthreads();
-    int wex = 0;
+    uint64_t mask = __ballot(places);
+    while (mask != 0) {
+      const int b = __builtin_ctzll(mask);
+      mask &= mask - 1;
+      const uint32_t tile = gw + (j0 + (uint32_t)b) * W;
+      int64_t gpos = readlane(lgt0, b), epos = readlane(leq0, b);  // the wave's running offsets
+      uint32_t subs = (uint32_t)readlane((int)lsub, b);
+      const int64_t t0 = (int64_t)tile * TILE;
+      auto batch = [&](int r, auto full_c) {
+        constexpr bool FULL = decltype(full_c)::value;
+        Packed<Raw, G> xv[B];
 #pragma unroll
-    for (int w = 0; w < NW; ++w) wex += w < wid ? s_wval[w] : 0;
-    __syncthreads();  // s_wval is rewritten for the next tile
-    const int64_t wbase = (int64_t)tile * TILE + (int64_t)wid * WSEG + (int64_t)lane * G;
+        for (int q = 0; q < B; ++q) load_items<Raw, G, FULL>(x, t0 + (r + q) * GSPAN + lane * G, n, xv[q]);
 #pragma unroll
-    for (int j = 0; j < J; ++j) {
-      const int before = wex + gex[j];
-      int64_t gpos = gt0 + (before >> 16), epos = eq0 + (before & 0xffff);
-      const int64_t gb = wbase + j * GSPAN;
+        for (int q = 0; q < B; ++q) {
+          const int64_t gb = t0 + (r + q) * GSPAN + lane * G;
+          uint32_t gtb = 0, eqb = 0;
 #pragma unroll
-      for (int i = 0; i < G; ++i) {
-        if ((gtb[j] >> i) & 1u) {
-          cand_v[gpos] = xv[j][i];
-          cand_i[gpos] = (int32_t)(gb + i);
-          ++gpos;
-        } else if ((eqb[j] >> i) & 1u) {
-          if (epos < need) {
-            cand_v[c_gt + epos] = xv[j][i];
-            cand_i[c_gt + epos] = (int32_t)(gb + i);
+          for (int i = 0; i < G; ++i) {
+            const uint32_t kk = KT::key(xv[q][i]) ^ flip;
+            const bool valid = FULL || gb + i < n;
+            gtb |= (valid && kk > T ? 1u : 0u) << i;
+            eqb |= (valid && kk == T ? 1u : 0u) << i;
           }
-          ++epos;
+          // Most groups of a placing tile hold no selected key: skip their
+          // scan (2^28 fp16, k = 2048: the scan and placement loop of every
+          // group made each placing wave a chain of ~5,400 VALU
+          // instructions, 53 us per call).
+          if (__ballot((gtb | eqb) != 0) == 0) continue;
+          const int c = (__builtin_popcount(gtb) << 16) | __builtin_popcount(eqb);
+          const int incl = wave_incl_scan(c);
+          const int tot = readlane(incl, kWave - 1);
+          if ((gtb | eqb) != 0) {
+            const int ex = incl - c;
+            int64_t gp = gpos + (ex >> 16), ep = epos + (ex & 0xffff);
+#pragma unroll
+            for (int i = 0; i < G; ++i) {
+              if ((gtb >> i) & 1u) {
+                cand_v[gp] = xv[q][i];
+                cand_i[gp] = (int32_t)(gb + i);
+                ++gp;
+              } else if ((eqb >> i) & 1u) {
+                if (ep < need) {
+                  cand_v[c_gt + ep] = xv[q][i];
+                  cand_i[c_gt + ep] = (int32_t)(gb + i);
+                }
+                ++ep;
+              }
+            }
+          }
+          gpos += tot >> 16;
+          epos += tot & 0xffff;
+        }
+      };
+      // Sub-chunks in order; the ones skipped hold no selected key.
+      const bool full_tile = t0 + TILE <= n;
+      while (subs != 0) {
+        const int c = __builtin_ctz(subs);
+        subs &= subs - 1;
+        if (full_tile) {
+          batch(c * B, std::integral_constant<bool, true>{});
+        } else if (t0 + c * kTkSubChunk < n) {
+          batch(c * B, std::integral_constant<bool, false>{});
         }
       }
     }
@@ -415,6 +621,66 @@ __global__ __launch_bounds__(256) void topk_pivot_kernel(const uint16_t* __restr
   if (tid == 0) out[blockIdx.x] = best_r;
 }
 
+// Orders k <= kTkSmallSort selected keys in one workgroup: a bitonic sort
+// in LDS of 64-bit words (inverted order key << 32 | candidate index), so
+// keys go largest first by the order key after flip and equal keys keep
+// candidate order, which is input order: the order the stable radix sort
+// gives, in one launch instead of five (2^28 fp16, k = 2048: 36.8 us for the
+// radix sort's launches).
+constexpr int kTkSmallSort = 4096, kTkSortThreads = 1024;
+template <int DT>
+__global__ __launch_bounds__(kTkSortThreads) void topk_small_sort_kernel(
+    const typename SortKey<DT>::Raw* __restrict__ cand_v, const int32_t* __restrict__ cand_i, int k, uint32_t flip,
+    typename SortKey<DT>::Raw* __restrict__ out_v, int32_t* __restrict__ out_i) {
+  using KT = SortKey<DT>;
+  __shared__ uint64_t s[kTkSmallSort];
+  const int tid = threadIdx.x;
+  int P = 1;
+  while (P < k) P <<= 1;
+  for (int e = tid; e < P; e += kTkSortThreads)
+    s[e] = e < k ? ((uint64_t)(~(KT::key(cand_v[e]) ^ flip)) << 32) | (uint32_t)e : ~0ull;
+  __syncthreads();
+  for (int size = 2; size <= P; size <<= 1) {
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int t = tid; t < P / 2; t += kTkSortThreads) {
+        const int lo = 2 * t - (t & (stride - 1));  // the pair (lo, lo + stride), bit `stride` of lo clear
+        const bool up = (lo & size) == 0;            // this bitonic block ascends
+        const uint64_t a = s[lo], b = s[lo + stride];
+        if ((a > b) == up) {
+          s[lo] = b;
+          s[lo + stride] = a;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (int e = tid; e < k; e += kTkSortThreads) {
+    const uint32_t c = (uint32_t)s[e];
+    out_v[e] = cand_v[c];
+    out_i[e] = cand_i[c];
+  }
+}
+
+// Placement walks the tiles with the resident grid of topk_place_kernel:
+// most tiles place nothing when k is small, and each that does costs a few
+// dependent round trips to memory, which the resident waves pay in
+// parallel.  2^28 fp16, k = 2048: a workgroup of 8 waves per tile (block
+// scans between the waves) took 42 us with 2048 workgroups of which about
+// 512 fit at once and 52 us with the resident grid.
+template <int DT>
+static uint32_t topk_place_grid() {
+  static std::atomic<int> cached[kMaxCachedDevices];
+  return (uint32_t)device_cached(cached, [](int dev) {
+    int per_cu = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, topk_place_kernel<DT>, kTkPlaceThreads, 0) !=
+            hipSuccess ||
+        per_cu < 1)
+      per_cu = 1;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 256;
+    return per_cu * cus;
+  });
+}
+
 inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
 inline int topk_elem_bytes(int dt) {
@@ -428,7 +694,7 @@ inline int topk_elem_bytes(int dt) {
 
 struct TopkLayout {
   size_t state_off, part_off, psum_off, own_total;
-  size_t cand_i_off, tcnt_off, sort_scratch_off, scratch;
+  size_t cand_i_off, tcnt_off, tstat_off, tlist_off, sort_scratch_off, scratch;
   int groups;
   int64_t chunk;
 };
@@ -439,8 +705,11 @@ inline TopkLayout topk_layout(int64_t n, int64_t k, int dt) {
   int64_t groups = (n + 65535) / 65536;
   if (groups > kTkMaxGroups) groups = kTkMaxGroups;
   if (groups < 1) groups = 1;
+  // Histogram chunks are whole tiles (the last round records per tile).
+  const int64_t tile = topk_tile(topk_elem_bytes(dt));
   int64_t chunk = (n + groups - 1) / groups;
-  chunk = (chunk + 63) & ~(int64_t)63;
+  chunk = (chunk + tile - 1) / tile * tile;
+  groups = n > 0 ? (n + chunk - 1) / chunk : 1;
   L.groups = (int)groups;
   L.chunk = chunk;
   L.state_off = lookback_ws_bytes(tiles);
@@ -449,7 +718,9 @@ inline TopkLayout topk_layout(int64_t n, int64_t k, int dt) {
   L.own_total = align256(L.psum_off + (size_t)kTkSumGroups * 256 * 8);
   L.cand_i_off = align256((size_t)k * topk_elem_bytes(dt));
   L.tcnt_off = align256(L.cand_i_off + (size_t)k * 4);
-  L.sort_scratch_off = align256(L.tcnt_off + (size_t)(tiles + 1) * 8);
+  L.tstat_off = align256(L.tcnt_off + (size_t)(tiles + 1) * 8);
+  L.tlist_off = align256(L.tstat_off + (size_t)tiles * 8);
+  L.sort_scratch_off = align256(L.tlist_off + (size_t)tiles * kTkListMax);
   L.scratch = L.sort_scratch_off + tcs_sort_scratch_bytes(k, dt);
   return L;
 }
@@ -472,9 +743,22 @@ static int topk_run(const void* x, int64_t n, int64_t k, bool largest, void* out
   auto* cand_i = reinterpret_cast<int32_t*>(sc + L.cand_i_off);
   const uint32_t flip = largest ? 0u : (KT::kBits == 32 ? 0xffffffffu : ((1u << KT::kBits) - 1u));
   const Raw* xr = static_cast<const Raw*>(x);
+  constexpr int64_t kTile = TopkGeom<DT>::kTile;
+  static_assert(kTile == (int64_t)kTkNW * (sizeof(Raw) <= 2 ? 4 : 2) * kWave * kTkG, "topk_tile");
+  if (L.chunk % kTile != 0 || L.chunk / kTile > kTkMaxChunkTiles) return TCS_ERR_INVALID_ARG;
+  const uint32_t tiles = (uint32_t)((n + kTile - 1) / kTile);
+  auto* tcnt = reinterpret_cast<uint64_t*>(sc + L.tcnt_off);
+  auto* tstat = reinterpret_cast<uint64_t*>(sc + L.tstat_off);
+  auto* tlist = reinterpret_cast<uint8_t*>(sc + L.tlist_off);
+  constexpr bool kFuse = P > 1;  // keys of one byte: the tile counts read the keys
   for (int r = P - 1; r >= 0; --r) {
-    hipLaunchKernelGGL((topk_hist_kernel<DT>), dim3(L.groups), dim3(kTkThreads), 0, st, xr, n, L.chunk, flip, r,
-                       state, partial);
+    if (kFuse && r == 0) {
+      hipLaunchKernelGGL((topk_hist_kernel<DT, kFuse>), dim3(L.groups), dim3(kTkThreads), 0, st, xr, n, L.chunk,
+                         flip, r, state, partial, tstat, tlist);
+    } else {
+      hipLaunchKernelGGL((topk_hist_kernel<DT, false>), dim3(L.groups), dim3(kTkThreads), 0, st, xr, n, L.chunk,
+                         flip, r, state, partial, nullptr, nullptr);
+    }
     TCS_CHECK_LAUNCH();
     hipLaunchKernelGGL(topk_sum_kernel, dim3(sum_groups), dim3(256), 0, st, partial, L.groups, sum_per, psum);
     TCS_CHECK_LAUNCH();
@@ -482,19 +766,31 @@ static int topk_run(const void* x, int64_t n, int64_t k, bool largest, void* out
                        r == P - 1 ? 1 : 0, k, state);
     TCS_CHECK_LAUNCH();
   }
-  constexpr int64_t kTile = TopkGeom<DT>::kTile;
-  const uint32_t tiles = (uint32_t)((n + kTile - 1) / kTile);
-  auto* tcnt = reinterpret_cast<uint64_t*>(sc + L.tcnt_off);
-  hipLaunchKernelGGL((topk_tilecount_kernel<DT>), dim3(tiles), dim3(kTkNW * kWave), 0, st, xr, n, flip, state, tcnt);
+  if constexpr (kFuse) {
+    hipLaunchKernelGGL((topk_tilefix_kernel<DT>), dim3((tiles + 3) / 4), dim3(256), 0, st, xr, n, flip, state, tstat,
+                       tlist, tcnt, tiles);
+  } else {
+    hipLaunchKernelGGL((topk_tilecount_kernel<DT>), dim3(tiles), dim3(kTkNW * kWave), 0, st, xr, n, flip, state,
+                       tcnt);
+  }
   TCS_CHECK_LAUNCH();
   hipLaunchKernelGGL(topk_tilescan_kernel, dim3(1), dim3(kTkScanThreads), 0, st, tcnt, tiles);
   TCS_CHECK_LAUNCH();
-  const uint32_t place_grid = tiles < kTkPlaceGrid ? tiles : kTkPlaceGrid;
-  hipLaunchKernelGGL((topk_place_kernel<DT>), dim3(place_grid), dim3(kTkNW * kWave), 0, st, xr, n, flip, state, k,
-                     tcnt, cand_v, cand_i, tiles);
+  const uint32_t resident = topk_place_grid<DT>();
+  const uint32_t wave_groups = (tiles + kTkPlaceThreads / kWave - 1) / (kTkPlaceThreads / kWave);  // a wave per tile
+  const uint32_t place_grid = wave_groups < resident ? wave_groups : resident;
+  hipLaunchKernelGGL((topk_place_kernel<DT>), dim3(place_grid), dim3(kTkPlaceThreads), 0, st, xr, n, flip, state, k,
+                     tcnt, kFuse ? tstat : nullptr, cand_v, cand_i, tiles);
   TCS_CHECK_LAUNCH();
   // Order the k selected: largest first (smallest first for !largest), ties
-  // in input order.  The sort reuses this workspace (same header and epoch).
+  // in input order.  The radix sort reuses this workspace (same header and
+  // epoch).
+  if (k <= kTkSmallSort) {
+    hipLaunchKernelGGL((topk_small_sort_kernel<DT>), dim3(1), dim3(kTkSortThreads), 0, st, cand_v, cand_i, (int)k,
+                       flip, static_cast<Raw*>(out_v), out_i);
+    TCS_CHECK_LAUNCH();
+    return TCS_OK;
+  }
   return tcs_sort(cand_v, DT, cand_i, out_v, out_i, k, largest ? 1 : 0, ws, ws_bytes, sc + L.sort_scratch_off,
                   scratch_bytes - L.sort_scratch_off, st);
 }

@@ -113,7 +113,8 @@ def _special(n, dtype, seed):
 def test_topk_every_dtype(ops, dev, dtype, largest):
     for n in (1, 100, 8192, 8193, 50_001, 400_000):
         x = _special(n, dtype, n)
-        for k in sorted({1, min(n, 3), min(n, 1000), n}):
+        # 4096 and 4097: the largest k sorted in one workgroup, the smallest by the radix sort
+        for k in sorted({1, min(n, 3), min(n, 1000), min(n, 4096), min(n, 4097), n}):
             v, i = ops.run_topk(x.to(dev), k, largest)
             if dtype == torch.bfloat16:
                 ev, ei = oracle.topk(x.float().numpy(), k, largest)
@@ -123,6 +124,34 @@ def test_topk_every_dtype(ops, dev, dtype, largest):
             ev, ei = oracle.topk(to_np(x), k, largest)
             assert_exact(to_np(i), ei, f"n={n} k={k} indices")
             assert_exact(to_np(v).view(np.uint8), ev.view(np.uint8), f"n={n} k={k} values")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dtype", [torch.float16, torch.float32, torch.int16], ids=str)
+@pytest.mark.parametrize("largest", [True, False])
+def test_topk_recorded_and_recounted_tiles(ops, dev, dtype, largest):
+    """Keys of two or more bytes: the last digit round records, per tile, the
+    keys above the chosen higher digits and the low bytes of up to 16 keys
+    matching them, and the tile counts come from that record.  Normal keys
+    with 1,100 copies of one value near the k-th largest planted in a few
+    tiles: most tiles take their recorded counts, the planted ones (more
+    than 16 matching keys) are counted again from their keys, and the ties at
+    the k-th value are taken in input order."""
+    n = (1 << 20) + 333
+    if dtype.is_floating_point:
+        x = torch.randn(n, generator=gen(77)).to(dtype)
+        v0 = 3.6 if largest else -3.6
+    else:
+        x = (torch.randn(n, generator=gen(77)) * 2000).round().clamp(-32768, 32767).to(dtype)
+        v0 = 7200 if largest else -7200
+    x[200_000:201_000] = v0
+    x[700_000:700_500:5] = v0
+    xd = x.to(dev)
+    for k in (1, 100, 600, 1500, 3000):
+        v, i = ops.run_topk(xd, k, largest)
+        ev, ei = oracle.topk(to_np(x), k, largest)
+        assert_exact(to_np(i), ei, f"k={k} indices")
+        assert_exact(to_np(v).view(np.uint8), ev.view(np.uint8), f"k={k} values")
 
 
 @pytest.mark.gpu
