@@ -622,42 +622,101 @@ __global__ __launch_bounds__(256) void topk_pivot_kernel(const uint16_t* __restr
 }
 
 // Orders k <= kTkSmallSort selected keys in one workgroup: a bitonic sort
-// in LDS of 64-bit words (inverted order key << 32 | candidate index), so
-// keys go largest first by the order key after flip and equal keys keep
-// candidate order, which is input order: the order the stable radix sort
-// gives, in one launch instead of five (2^28 fp16, k = 2048: 36.8 us for the
-// radix sort's launches).
+// of 64-bit words (inverted order key << 32 | candidate index), so keys go
+// largest first by the order key after flip and equal keys keep candidate
+// order, which is input order: the order the stable radix sort gives, in one
+// launch instead of five.  The workgroup sorts P = 1024 E words; lane l of
+// wave w holds positions w * 64E + 64e + l, e < E, in registers, so the
+// compare-exchange steps of stride below 64 are lane shuffles, strides 64 to
+// 32E are swaps inside a lane, and only the larger strides go through LDS
+// with barriers (2^28 fp16, k = 2048: 36.8 us for the radix sort's five
+// launches, 21.7 us with every step in LDS).
 constexpr int kTkSmallSort = 4096, kTkSortThreads = 1024;
-template <int DT>
+template <int DT, int E>
 __global__ __launch_bounds__(kTkSortThreads) void topk_small_sort_kernel(
     const typename SortKey<DT>::Raw* __restrict__ cand_v, const int32_t* __restrict__ cand_i, int k, uint32_t flip,
     typename SortKey<DT>::Raw* __restrict__ out_v, int32_t* __restrict__ out_i) {
   using KT = SortKey<DT>;
-  __shared__ uint64_t s[kTkSmallSort];
-  const int tid = threadIdx.x;
-  int P = 1;
-  while (P < k) P <<= 1;
-  for (int e = tid; e < P; e += kTkSortThreads)
-    s[e] = e < k ? ((uint64_t)(~(KT::key(cand_v[e]) ^ flip)) << 32) | (uint32_t)e : ~0ull;
-  __syncthreads();
-  for (int size = 2; size <= P; size <<= 1) {
-    for (int stride = size >> 1; stride > 0; stride >>= 1) {
-      for (int t = tid; t < P / 2; t += kTkSortThreads) {
-        const int lo = 2 * t - (t & (stride - 1));  // the pair (lo, lo + stride), bit `stride` of lo clear
-        const bool up = (lo & size) == 0;            // this bitonic block ascends
-        const uint64_t a = s[lo], b = s[lo + stride];
-        if ((a > b) == up) {
-          s[lo] = b;
-          s[lo + stride] = a;
+  constexpr int P = kTkSortThreads * E, SEG = kWave * E;
+  static_assert(E == 2 || E == 4, "two or four words per lane");
+  __shared__ uint64_t s[P];
+  const int tid = threadIdx.x, lane = __lane_id();
+  const int seg = (tid / kWave) * SEG + lane;  // position of word 0
+  uint64_t a[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int pos = seg + e * kWave;
+    a[e] = pos < k ? ((uint64_t)(~(KT::key(cand_v[pos]) ^ flip)) << 32) | (uint32_t)pos : ~0ull;
+  }
+  // Swaps inside the lane: words e and e + D (stride 64 D).
+  auto cas_regs = [&](int size, auto d_c) {
+    constexpr int D = decltype(d_c)::value;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      if ((e & D) == 0) {
+        const bool up = ((seg + e * kWave) & size) == 0;
+        const uint64_t x0 = a[e], x1 = a[e + D];
+        if ((x0 > x1) == up) {
+          a[e] = x1;
+          a[e + D] = x0;
         }
       }
+    }
+  };
+  for (int size = 2; size <= P; size <<= 1) {
+    int stride = size >> 1;
+    if (stride >= SEG) {
+#pragma unroll
+      for (int e = 0; e < E; ++e) s[seg + e * kWave] = a[e];
       __syncthreads();
+      for (; stride >= SEG; stride >>= 1) {
+        for (int t = tid; t < P / 2; t += kTkSortThreads) {
+          const int lo = 2 * t - (t & (stride - 1));  // the pair (lo, lo + stride), bit `stride` of lo clear
+          const bool up = (lo & size) == 0;            // this bitonic block ascends
+          const uint64_t x0 = s[lo], x1 = s[lo + stride];
+          if ((x0 > x1) == up) {
+            s[lo] = x1;
+            s[lo + stride] = x0;
+          }
+        }
+        __syncthreads();
+      }
+      // Each thread reads back only its own positions, which no thread
+      // touches again before the barrier of the next LDS phase.
+#pragma unroll
+      for (int e = 0; e < E; ++e) a[e] = s[seg + e * kWave];
+    }
+    if constexpr (E == 4) {
+      if (stride == 2 * kWave) {
+        cas_regs(size, std::integral_constant<int, 2>{});
+        stride >>= 1;
+      }
+    }
+    if (stride == kWave) {
+      cas_regs(size, std::integral_constant<int, 1>{});
+      stride >>= 1;
+    }
+    for (; stride > 0; stride >>= 1) {
+      const bool lo = (lane & stride) == 0;
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const bool up = ((seg + e * kWave) & size) == 0;
+        const uint32_t blo = (uint32_t)__shfl_xor((int)(uint32_t)a[e], stride);
+        const uint32_t bhi = (uint32_t)__shfl_xor((int)(uint32_t)(a[e] >> 32), stride);
+        const uint64_t b = ((uint64_t)bhi << 32) | blo;
+        const uint64_t mn = a[e] < b ? a[e] : b, mx = a[e] < b ? b : a[e];
+        a[e] = lo == up ? mn : mx;
+      }
     }
   }
-  for (int e = tid; e < k; e += kTkSortThreads) {
-    const uint32_t c = (uint32_t)s[e];
-    out_v[e] = cand_v[c];
-    out_i[e] = cand_i[c];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int pos = seg + e * kWave;
+    if (pos < k) {
+      const uint32_t c = (uint32_t)a[e];
+      out_v[pos] = cand_v[c];
+      out_i[pos] = cand_i[c];
+    }
   }
 }
 
@@ -786,8 +845,13 @@ static int topk_run(const void* x, int64_t n, int64_t k, bool largest, void* out
   // in input order.  The radix sort reuses this workspace (same header and
   // epoch).
   if (k <= kTkSmallSort) {
-    hipLaunchKernelGGL((topk_small_sort_kernel<DT>), dim3(1), dim3(kTkSortThreads), 0, st, cand_v, cand_i, (int)k,
-                       flip, static_cast<Raw*>(out_v), out_i);
+    if (k <= kTkSortThreads * 2) {
+      hipLaunchKernelGGL((topk_small_sort_kernel<DT, 2>), dim3(1), dim3(kTkSortThreads), 0, st, cand_v, cand_i,
+                         (int)k, flip, static_cast<Raw*>(out_v), out_i);
+    } else {
+      hipLaunchKernelGGL((topk_small_sort_kernel<DT, 4>), dim3(1), dim3(kTkSortThreads), 0, st, cand_v, cand_i,
+                         (int)k, flip, static_cast<Raw*>(out_v), out_i);
+    }
     TCS_CHECK_LAUNCH();
     return TCS_OK;
   }
